@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Benchmark of the ffts R2C+C2R hot path on one MI355X.
+
+One step = one forward R2C + one backward C2R (unnormalised) of a 512^3 FP32 field
+already resident in HBM: the headline workload of the metric
+"R2C+C2R GFLOP/s & HBM GB/s at 512^3/64^4".  Protocol after
+source/tests/test_benchmark_ffts_3D.cu:65-121: W untimed warmup pairs, then K timed
+pairs bracketed by device synchronisation.
+
+Prints ONE JSON line.  ``value`` is GFLOP/s with the 2.5 N log2 N per-R2C convention
+(5 N log2 N per pair).  Also reported:
+  * ``roofline``: the dominant pass kernel's algorithmic bytes per launch over its
+    average duration (HIP events on the transform's stream over the timed region),
+    against 8 TB/s;
+  * ``hbm_gbps_alg``: B_alg = 2 [(4N + 8N_c) + (d-1) 16 N_c] bytes per pair / pair time;
+  * errors of the timed transform against the FP64 oracle (forward spectrum) and the
+    round trip;
+  * the 64^4 pencil-layout 4D pair (GFLOP/s, HBM GB/s);
+  * ``cpu_baseline``: the FP64 CPU oracle timed on this host's cores on a bounded
+    sample of the same 512^3 pair.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+SEED = 0x1234ABCDEF987654  # test_benchmark_ffts_3D.cu:83
+
+
+def _flops_pair(N):
+    return 5.0 * N * math.log2(N)
+
+
+def _b_alg(dims, elem_real):
+    N = int(np.prod(dims))
+    Nc = N // dims[-1] * (dims[-1] // 2 + 1)
+    d = len(dims)
+    return 2 * ((elem_real * N + 2 * elem_real * Nc) + (d - 1) * 4 * elem_real * Nc)
+
+
+def _traffic_summary(kernel_key):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        v = data.get(kernel_key)
+        return None if v is None else float(v["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def run_gpu(dims, strategy, steps, warmup, check):
+    import torch
+    import fftm_amd
+    from fftm_amd import _lib
+    import ctypes
+
+    dt = np.float32
+    f = fftm_amd.ffts("float32", strategy_4d=strategy)
+    opts = fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", execution="fused", sync=False)
+    f.init(*dims, options=opts)
+    stream = torch.cuda.current_stream()
+    f.set_stream(stream)
+    h = f._handle
+    L = _lib.lib()
+
+    # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor), resident in HBM
+    import oracle
+    if len(dims) == 3:
+        x_host = oracle.fill_random_real_3d(*dims, SEED, dt)
+    else:
+        x_host = oracle.to_memory(oracle.fill_random_real_4d_logical(*dims, SEED, dt),
+                                  oracle.A0123 if strategy.family.value == "pencil_pencil" else oracle.A3210)
+    u = torch.from_numpy(x_host).cuda()
+    X = f.complex_array()
+    y = f.real_array()
+    nf, nb = f.num_passes()
+    passes = [(0, i) for i in range(nf)] + [(1, i) for i in range(nb)]
+    pbytes = {}
+    for bw, i in passes:
+        b = ctypes.c_int64()
+        _lib.check(L.fftm_mi_ffts_pass_bytes(h, bw, i, ctypes.byref(b)), "pass_bytes")
+        pbytes[(bw, i)] = int(b.value)
+
+    def pair():
+        f.forward(u, X)
+        f.backward(X, y)
+
+    for _ in range(warmup):
+        pair()
+    torch.cuda.synchronize()
+
+    # timed region: K pairs; HIP events on the transform's stream between the passes
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(passes) + 1)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ev[s][0].record(stream)
+        for k, (bw, i) in enumerate(passes):
+            src, dst = (u, X) if bw == 0 else (X, y)
+            _lib.check(L.fftm_mi_ffts_run_pass(h, bw, i, src.data_ptr(), dst.data_ptr(), 0), "run_pass")
+            ev[s][k + 1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pair_s = (t1 - t0) / steps
+    per_pass_ms = {p: float(np.mean([ev[s][k].elapsed_time(ev[s][k + 1]) for s in range(steps)]))
+                   for k, p in enumerate(passes)}
+    # also the plain call (forward + backward API), no events in between
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        pair()
+    torch.cuda.synchronize()
+    api_pair_s = (time.perf_counter() - t2) / steps
+
+    res = {"pair_s": pair_s, "api_pair_s": api_pair_s, "per_pass_ms": per_pass_ms, "pass_bytes": pbytes,
+           "passes": passes}
+    if check:
+        pair()
+        torch.cuda.synchronize()
+        yr = y.cpu().numpy().astype(np.float64) / float(np.prod(dims))
+        d = yr - x_host.astype(np.float64)
+        res["roundtrip_rms"] = float(np.sqrt(np.mean(d * d)))
+        res["roundtrip_max"] = float(np.max(np.abs(d)))
+        f.forward(u, X)
+        torch.cuda.synchronize()
+        Xg = X.cpu().numpy()
+        if len(dims) == 3:
+            Xo = oracle.ffts3d_forward(x_host.astype(np.float64))
+        else:
+            Xo = oracle.ffts4d_forward(x_host.astype(np.float64), "pencil", *dims)
+        rms, mx = oracle.rel_errors(Xg, Xo)
+        res["fwd_rel_l2_vs_fp64"] = rms
+        res["fwd_rel_max_vs_fp64"] = mx
+        del Xo, Xg
+    f.close()
+    del u, X, y
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_cpu_baseline(dims, pairs=2, warmup=1):
+    import oracle
+    L = oracle.lib()
+    threads = int(L.fo_max_threads())
+    x = oracle.fill_random_real_3d(*dims, SEED, np.float64)
+    N = int(np.prod(dims))
+    for _ in range(warmup):
+        X = oracle.ffts3d_forward(x)
+        oracle.ffts3d_backward(X, dims[2])
+    t0 = time.perf_counter()
+    for _ in range(pairs):
+        X = oracle.ffts3d_forward(x)
+        oracle.ffts3d_backward(X, dims[2])
+    dt = (time.perf_counter() - t0) / pairs
+    return {"value": _flops_pair(N) / dt / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"{warmup} warmup + {pairs} timed R2C+C2R pairs of the same {dims[0]}x{dims[1]}x{dims[2]} "
+                      f"splitmix64 field in FP64 (oracle/fft_oracle.c, OpenMP {threads} threads); "
+                      f"{dt * 1e3:.1f} ms per pair",
+            "ms_per_pair": dt * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-4d", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (the transform runs on one device); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fftm_amd
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    n = args.size
+    dims = (n, n, n)
+    N = n ** 3
+    r3 = run_gpu(dims, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check)
+    pair_s = r3["pair_s"]
+    gflops = _flops_pair(N) / pair_s / 1e9
+    balg = _b_alg(dims, 4)
+    # dominant pass kernel
+    dom = max(r3["passes"], key=lambda p: r3["per_pass_ms"][p])
+    dom_ms = r3["per_pass_ms"][dom]
+    dom_bytes = r3["pass_bytes"][dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    dom_name = f"{'backward' if dom[0] else 'forward'} pass {dom[1]}"
+    traffic = _traffic_summary(f"ffts3d_{n}_f32_{'bwd' if dom[0] else 'fwd'}{dom[1]}")
+
+    out = {
+        "metric": "R2C+C2R GFLOP/s & HBM GB/s at 512^3/64^4, mixed vs FP32 at ≤1e-5 RMS err",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(pair_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: splitmix64 uniform[-1,1) field, seed {SEED:#x} (fill_random_real_3d_functor)",
+        "config": {"workload": f"ffts 3D R2C forward + C2R inverse pair, {n}^3 float32, fused passes",
+                   "nx": n, "ny": n, "nz": n, "flop_convention": "2.5 N log2 N per R2C (5 N log2 N per pair)"},
+        "hbm_gbps_alg": round(balg / pair_s / 1e9, 1),
+        "b_alg_bytes": balg,
+        "api_ms_per_pair": round(r3["api_pair_s"] * 1e3, 4),
+        "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r3["per_pass_ms"].items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "avg_ms": round(dom_ms, 4)},
+    }
+    if not args.no_check:
+        out["errors_vs_fp64_oracle"] = {k: r3[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64",
+                                                           "roundtrip_rms", "roundtrip_max")}
+    if not args.no_4d:
+        d4 = (64, 64, 64, 64)
+        r4 = run_gpu(d4, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check)
+        N4 = 64 ** 4
+        out["secondary_64^4"] = {
+            "workload": "ffts 4D pencil layout R2C+C2R pair 64^4 float32 (real [w][z][y][x] -> [y][z][w/2+1][x])",
+            "gflops": round(_flops_pair(N4) / r4["pair_s"] / 1e9, 2),
+            "ms_per_pair": round(r4["pair_s"] * 1e3, 4),
+            "hbm_gbps_alg": round(_b_alg(d4, 4) / r4["pair_s"] / 1e9, 1),
+            "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r4["per_pass_ms"].items()},
+        }
+        if not args.no_check:
+            out["secondary_64^4"]["errors_vs_fp64_oracle"] = {
+                k: r4[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64", "roundtrip_rms", "roundtrip_max")}
+    out["mixed_precision"] = None
+    if not args.no_cpu:
+        out["cpu_baseline"] = run_cpu_baseline(dims)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

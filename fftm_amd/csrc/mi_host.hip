@@ -1,0 +1,1040 @@
+// Host side of the MI355X FFT backend: descriptor validation, decomposition of a
+// plan into HBM passes, pass execution, the fused ffts pipelines, the 4D
+// transposes, and the C ABI declared in include/fftm_mi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/fftm_mi.h"
+#include "mi_launch.hpp"
+#include "mi_transpose.hpp"
+
+namespace fftm_mi
+{
+namespace
+{
+
+thread_local std::string g_err;
+
+int fail( int code, const std::string &msg )
+{
+    g_err = msg;
+    return code;
+}
+
+int ok()
+{
+    return FFTM_MI_SUCCESS;
+}
+
+enum : int
+{
+    BUF_IN   = 0,
+    BUF_OUT  = 1,
+    BUF_WORK = 2
+};
+
+struct bdim
+{
+    int64_t n, is, os;
+};
+
+// One HBM pass: a batch of 1D transforms along one axis.
+struct pass_rt
+{
+    int        kind   = KIND_C2C;
+    bool       inv    = false;
+    int64_t    N      = 1;
+    int        src    = BUF_IN;
+    int        dst    = BUF_OUT;
+    int64_t    ies    = 1, oes = 1;
+    int64_t    ncols  = 0;
+    int64_t    d0 = 1, d1 = 1, is0 = 0, is1 = 0, is2 = 0, os0 = 0, os1 = 0, os2 = 0;
+    bool       loadb  = false, storeb = false;
+    bool       generic = false;
+    radix_list rl{};
+    int        gen_cols = 1;
+    double     scale    = 1.0;
+};
+
+bool is_pow2( int64_t n )
+{
+    return n >= 1 && ( n & ( n - 1 ) ) == 0;
+}
+
+// radices for the generic kernel (each <= 16); false if N has a prime factor > 13
+bool generic_radices( int64_t N, radix_list &rl )
+{
+    rl.n      = 0;
+    int64_t m = N;
+    int     k = 0;
+    while ( m % 2 == 0 )
+    {
+        m /= 2;
+        ++k;
+    }
+    while ( k >= 4 )
+    {
+        rl.r[rl.n++] = 16;
+        k -= 4;
+    }
+    if ( k == 3 )
+        rl.r[rl.n++] = 8;
+    else if ( k == 2 )
+        rl.r[rl.n++] = 4;
+    else if ( k == 1 )
+        rl.r[rl.n++] = 2;
+    const int primes[] = { 3, 5, 7, 11, 13 };
+    for ( int p : primes )
+        while ( m % p == 0 )
+        {
+            if ( rl.n >= 20 )
+                return false;
+            rl.r[rl.n++] = p;
+            m /= p;
+        }
+    return m == 1;
+}
+
+int csize_of( int prec )
+{
+    return prec == FFTM_MI_F32 ? 8 : 16;
+}
+
+// Decide how a pass of length N runs (fixed kernel or generic); fills N-dependent fields.
+int configure_kernel( pass_rt &p, int prec )
+{
+    if ( fixed_supported( p.N ) )
+    {
+        p.generic = false;
+        return ok();
+    }
+    if ( !generic_radices( p.N, p.rl ) )
+        return fail( FFTM_MI_NOT_SUPPORTED,
+                     "fftm_mi: transform length " + std::to_string( p.N ) +
+                         " has a prime factor above 13 (not supported yet)" );
+    const int cs   = csize_of( prec );
+    int       cols = prec == FFTM_MI_F32 ? 16 : 8;
+    while ( cols > 1 && 2 * p.N * cols * cs > 65536 )
+        cols /= 2;
+    if ( 2 * p.N * cols * cs > 65536 )
+        return fail( FFTM_MI_NOT_SUPPORTED,
+                     "fftm_mi: transform length " + std::to_string( p.N ) + " exceeds the single-pass limit" );
+    p.generic  = true;
+    p.gen_cols = cols;
+    return ok();
+}
+
+// Fill the batch decomposition of a pass from its batch dims.
+int set_batch( pass_rt &p, std::vector<bdim> dims )
+{
+    std::vector<bdim> v;
+    for ( const bdim &d : dims )
+        if ( d.n > 1 )
+            v.push_back( d );
+    p.loadb  = p.ies != 1;
+    p.storeb = p.oes != 1;
+    const bool by_out = !p.loadb && p.storeb;
+    std::stable_sort( v.begin(), v.end(), [by_out]( const bdim &a, const bdim &b ) {
+        return by_out ? a.os < b.os : a.is < b.is;
+    } );
+    std::vector<bdim> m;
+    for ( const bdim &d : v )
+    {
+        if ( !m.empty() && m.back().n * m.back().is == d.is && m.back().n * m.back().os == d.os )
+            m.back().n *= d.n;
+        else
+            m.push_back( d );
+    }
+    if ( m.size() > 3 )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi: pass needs more than 3 batch dimensions" );
+    while ( m.size() < 3 )
+        m.push_back( bdim{ 1, 0, 0 } );
+    p.ncols = m[0].n * m[1].n * m[2].n;
+    p.d0    = m[0].n;
+    p.d1    = m[1].n;
+    p.is0   = m[0].is;
+    p.is1   = m[1].is;
+    p.is2   = m[2].is;
+    p.os0   = m[0].os;
+    p.os1   = m[1].os;
+    p.os2   = m[2].os;
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// twiddle tables (one per distinct length, owned by the plan / ffts handle)
+// ---------------------------------------------------------------------------
+struct tables
+{
+    int                         prec = 0;
+    std::map<int64_t, void *>   by_n;
+
+    ~tables() { release(); }
+
+    void release()
+    {
+        for ( auto &kv : by_n )
+            if ( kv.second )
+                (void)hipFree( kv.second );
+        by_n.clear();
+    }
+
+    int ensure( int64_t N )
+    {
+        if ( by_n.count( N ) )
+            return ok();
+        const size_t cs = (size_t)csize_of( prec );
+        std::vector<unsigned char> host( cs * (size_t)N );
+        const long double two_pi = 6.28318530717958647692528676655900577L;
+        for ( int64_t k = 0; k < N; ++k )
+        {
+            const long double a = -two_pi * (long double)k / (long double)N;
+            const long double c = cosl( a ), s = sinl( a );
+            if ( prec == FFTM_MI_F32 )
+            {
+                float v[2] = { (float)c, (float)s };
+                std::memcpy( host.data() + cs * (size_t)k, v, sizeof( v ) );
+            }
+            else
+            {
+                double v[2] = { (double)c, (double)s };
+                std::memcpy( host.data() + cs * (size_t)k, v, sizeof( v ) );
+            }
+        }
+        void      *d = nullptr;
+        hipError_t e = hipMalloc( &d, host.size() );
+        if ( e != hipSuccess )
+            return fail( FFTM_MI_ALLOC_FAILED, std::string( "fftm_mi: hipMalloc(twiddles): " ) + hipGetErrorString( e ) );
+        e = hipMemcpy( d, host.data(), host.size(), hipMemcpyHostToDevice );
+        if ( e != hipSuccess )
+        {
+            (void)hipFree( d );
+            return fail( FFTM_MI_ALLOC_FAILED, std::string( "fftm_mi: twiddle upload: " ) + hipGetErrorString( e ) );
+        }
+        by_n[N] = d;
+        return ok();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// pass execution
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void *dst, hipStream_t s )
+{
+    pass_params<T> p;
+    p.in      = src;
+    p.out     = dst;
+    p.tw      = static_cast<const cpx<T> *>( tb.by_n.at( r.N ) );
+    p.ies     = r.ies;
+    p.oes     = r.oes;
+    p.ncols   = r.ncols;
+    p.d0      = r.d0;
+    p.d1      = r.d1;
+    p.is0     = r.is0;
+    p.is1     = r.is1;
+    p.is2     = r.is2;
+    p.os0     = r.os0;
+    p.os1     = r.os1;
+    p.os2     = r.os2;
+    p.store_b = r.storeb ? 1 : 0;
+    p.pad_    = 0;
+    p.scale   = (T)r.scale;
+    if constexpr ( std::is_same<T, float>::value )
+        return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
+                         : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, s );
+    else
+        return r.generic ? launch_generic_f64( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
+                         : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, s );
+}
+
+int run_passes( const std::vector<pass_rt> &passes, const tables &tb, int prec, const void *in, void *out,
+                void *work, hipStream_t s, bool flip_c2c, bool sync, const char *what )
+{
+    for ( const pass_rt &r0 : passes )
+    {
+        pass_rt r = r0;
+        if ( flip_c2c && r.kind == KIND_C2C )
+            r.inv = !r.inv;
+        const void *src = r.src == BUF_IN ? in : ( r.src == BUF_OUT ? (const void *)out : (const void *)work );
+        void       *dst = r.dst == BUF_OUT ? out : ( r.dst == BUF_WORK ? work : const_cast<void *>( in ) );
+        hipError_t  e   = prec == FFTM_MI_F32 ? launch_one<float>( r, tb, src, dst, s )
+                                              : launch_one<double>( r, tb, src, dst, s );
+        if ( e != hipSuccess )
+            return fail( FFTM_MI_EXEC_FAILED,
+                         std::string( "fftm_mi: " ) + what + ": kernel launch failed: " + hipGetErrorString( e ) );
+    }
+    if ( sync )
+    {
+        const hipError_t e = hipDeviceSynchronize();
+        if ( e != hipSuccess )
+            return fail( FFTM_MI_EXEC_FAILED,
+                         std::string( "fftm_mi: " ) + what + ": device synchronization failed: " + hipGetErrorString( e ) );
+    }
+    return ok();
+}
+
+int prepare_passes( std::vector<pass_rt> &passes, int prec, tables *tb )
+{
+    for ( pass_rt &p : passes )
+    {
+        const int rc = configure_kernel( p, prec );
+        if ( rc != FFTM_MI_SUCCESS )
+            return rc;
+        if ( tb )
+        {
+            const int rt = tb->ensure( p.N );
+            if ( rt != FFTM_MI_SUCCESS )
+                return rt;
+        }
+    }
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// plan decomposition (hipFFT advanced layout semantics)
+// ---------------------------------------------------------------------------
+struct plan_geom
+{
+    int     rank;
+    int64_t n[3];
+    int64_t ext_in[3], ext_out[3];
+    int64_t emb_in[3], emb_out[3];
+    int64_t istride, idist, ostride, odist, batch;
+    int64_t pin[3], pout[3]; // element strides of each axis
+};
+
+int64_t span_of( int rank, const int64_t *ext, const int64_t *pitch, int64_t dist, int64_t batch )
+{
+    int64_t off = ( batch - 1 ) * dist;
+    for ( int k = 0; k < rank; ++k )
+        off += ( ext[k] - 1 ) * pitch[k];
+    return off + 1;
+}
+
+int geom_from_desc( const fftm_mi_plan_desc &d, plan_geom &g )
+{
+    if ( d.rank < 1 || d.rank > 3 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi: rank must be 1, 2 or 3" );
+    if ( d.dir < FFTM_MI_R2C || d.dir > FFTM_MI_C2CB )
+        return fail( FFTM_MI_INVALID_TYPE, "fftm_mi: unknown transform direction" );
+    if ( d.precision != FFTM_MI_F32 && d.precision != FFTM_MI_F64 )
+        return fail( FFTM_MI_INVALID_TYPE, "fftm_mi: unknown precision" );
+    if ( d.batch < 1 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi: batch must be >= 1" );
+    g.rank  = d.rank;
+    g.batch = d.batch;
+    for ( int k = 0; k < d.rank; ++k )
+    {
+        if ( d.n[k] < 1 )
+            return fail( FFTM_MI_INVALID_SIZE, "fftm_mi: transform sizes must be >= 1" );
+        g.n[k]       = d.n[k];
+        g.ext_in[k]  = d.n[k];
+        g.ext_out[k] = d.n[k];
+    }
+    const int     L  = d.rank - 1;
+    const int64_t nh = d.n[L] / 2 + 1;
+    if ( d.dir == FFTM_MI_R2C )
+        g.ext_out[L] = nh;
+    if ( d.dir == FFTM_MI_C2R )
+        g.ext_in[L] = nh;
+    if ( d.default_layout )
+    {
+        g.istride = g.ostride = 1;
+        g.idist = g.odist = 1;
+        for ( int k = 0; k < d.rank; ++k )
+        {
+            g.emb_in[k]  = g.ext_in[k];
+            g.emb_out[k] = g.ext_out[k];
+            g.idist *= g.ext_in[k];
+            g.odist *= g.ext_out[k];
+        }
+    }
+    else
+    {
+        g.istride = d.istride;
+        g.ostride = d.ostride;
+        g.idist   = d.idist;
+        g.odist   = d.odist;
+        if ( g.istride < 1 || g.ostride < 1 )
+            return fail( FFTM_MI_INVALID_VALUE, "fftm_mi: istride/ostride must be >= 1" );
+        if ( d.batch > 1 && ( g.idist < 1 || g.odist < 1 ) )
+            return fail( FFTM_MI_INVALID_VALUE, "fftm_mi: idist/odist must be >= 1 for batch > 1" );
+        for ( int k = 0; k < d.rank; ++k )
+        {
+            g.emb_in[k]  = d.inembed[k];
+            g.emb_out[k] = d.onembed[k];
+            if ( k > 0 && ( g.emb_in[k] < g.ext_in[k] || g.emb_out[k] < g.ext_out[k] ) )
+                return fail( FFTM_MI_INVALID_VALUE, "fftm_mi: inembed/onembed smaller than the transform extent" );
+        }
+    }
+    for ( int k = 0; k < d.rank; ++k )
+    {
+        int64_t pi = g.istride, po = g.ostride;
+        for ( int m = k + 1; m < d.rank; ++m )
+        {
+            pi *= g.emb_in[m];
+            po *= g.emb_out[m];
+        }
+        g.pin[k]  = pi;
+        g.pout[k] = po;
+    }
+    return ok();
+}
+
+// Build the pass list of one plan.  work_elems: complex elements of work needed.
+int build_plan_passes( const fftm_mi_plan_desc &d, std::vector<pass_rt> &out, int64_t &work_elems )
+{
+    plan_geom g;
+    int       rc = geom_from_desc( d, g );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    out.clear();
+    work_elems     = 0;
+    const int r    = g.rank;
+    const int L    = r - 1;
+    auto other_dims = [&]( int ax, const int64_t *ext, const int64_t *pi, const int64_t *po, int64_t di,
+                           int64_t dout ) {
+        std::vector<bdim> v;
+        for ( int k = 0; k < r; ++k )
+            if ( k != ax )
+                v.push_back( bdim{ ext[k], pi[k], po[k] } );
+        v.push_back( bdim{ g.batch, di, dout } );
+        return v;
+    };
+    if ( d.dir == FFTM_MI_C2CF || d.dir == FFTM_MI_C2CB )
+    {
+        const bool inv   = d.dir == FFTM_MI_C2CB;
+        bool       first = true;
+        for ( int ax = L; ax >= 0; --ax )
+        {
+            pass_rt p;
+            p.kind = KIND_C2C;
+            p.inv  = inv;
+            p.N    = g.n[ax];
+            if ( first )
+            {
+                p.src = BUF_IN;
+                p.dst = BUF_OUT;
+                p.ies = g.pin[ax];
+                p.oes = g.pout[ax];
+                rc    = set_batch( p, other_dims( ax, g.n, g.pin, g.pout, g.idist, g.odist ) );
+            }
+            else
+            {
+                p.src = BUF_OUT;
+                p.dst = BUF_OUT;
+                p.ies = p.oes = g.pout[ax];
+                rc            = set_batch( p, other_dims( ax, g.n, g.pout, g.pout, g.odist, g.odist ) );
+            }
+            if ( rc != FFTM_MI_SUCCESS )
+                return rc;
+            first = false;
+            out.push_back( p );
+        }
+        return ok();
+    }
+    if ( d.dir == FFTM_MI_R2C )
+    {
+        pass_rt p;
+        p.kind = KIND_R2C;
+        p.N    = g.n[L];
+        p.src  = BUF_IN;
+        p.dst  = BUF_OUT;
+        p.ies  = g.pin[L];
+        p.oes  = g.pout[L];
+        rc     = set_batch( p, other_dims( L, g.n, g.pin, g.pout, g.idist, g.odist ) );
+        if ( rc != FFTM_MI_SUCCESS )
+            return rc;
+        out.push_back( p );
+        for ( int ax = L - 1; ax >= 0; --ax )
+        {
+            pass_rt q;
+            q.kind = KIND_C2C;
+            q.N    = g.n[ax];
+            q.src = q.dst = BUF_OUT;
+            q.ies = q.oes = g.pout[ax];
+            rc            = set_batch( q, other_dims( ax, g.ext_out, g.pout, g.pout, g.odist, g.odist ) );
+            if ( rc != FFTM_MI_SUCCESS )
+                return rc;
+            out.push_back( q );
+        }
+        return ok();
+    }
+    // C2R: inverse C2C along axes 0..L-1 (into work, or in place on the input), then C2R along L
+    const bool overwrite = ( d.flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0;
+    const int  scratch   = overwrite ? BUF_IN : BUF_WORK;
+    if ( L > 0 && !overwrite )
+        work_elems = span_of( r, g.ext_in, g.pin, g.idist, g.batch );
+    bool first = true;
+    for ( int ax = 0; ax < L; ++ax )
+    {
+        pass_rt q;
+        q.kind = KIND_C2C;
+        q.inv  = true;
+        q.N    = g.n[ax];
+        q.src  = first ? BUF_IN : scratch;
+        q.dst  = scratch;
+        q.ies = q.oes = g.pin[ax];
+        rc            = set_batch( q, other_dims( ax, g.ext_in, g.pin, g.pin, g.idist, g.idist ) );
+        if ( rc != FFTM_MI_SUCCESS )
+            return rc;
+        first = false;
+        out.push_back( q );
+    }
+    pass_rt p;
+    p.kind = KIND_C2R;
+    p.inv  = true;
+    p.N    = g.n[L];
+    p.src  = L > 0 ? scratch : BUF_IN;
+    p.dst  = BUF_OUT;
+    p.ies  = g.pin[L];
+    p.oes  = g.pout[L];
+    rc     = set_batch( p, other_dims( L, g.n, g.pin, g.pout, g.idist, g.odist ) );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    out.push_back( p );
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// fused ffts pipelines
+// ---------------------------------------------------------------------------
+// A strided 3-D/4-D array description: extents and element strides per logical axis.
+struct arr4
+{
+    int64_t n[4];
+    int64_t s[4];
+};
+
+// pass along logical axis `ax` between two arrays of the same logical extents
+// (for the real pass the complex side's extent of axis ax is N/2+1)
+pass_rt axis_pass( int kind, bool inv, int ax, int dim, const int64_t *ext, const int64_t *si, const int64_t *so,
+                   int src, int dst, int64_t N, int &rc )
+{
+    pass_rt p;
+    p.kind = kind;
+    p.inv  = inv;
+    p.N    = N;
+    p.src  = src;
+    p.dst  = dst;
+    p.ies  = si[ax];
+    p.oes  = so[ax];
+    std::vector<bdim> v;
+    for ( int k = 0; k < dim; ++k )
+        if ( k != ax )
+            v.push_back( bdim{ ext[k], si[k], so[k] } );
+    rc = set_batch( p, v );
+    return p;
+}
+
+struct ffts_layout
+{
+    int     dim;
+    int64_t n[4];    // logical extents (x, y, z[, w]); the last axis is the real one
+    int64_t sr[4];   // real array strides per logical axis
+    int64_t sc[4];   // spectrum strides per logical axis
+};
+
+int make_ffts_layout( int32_t layout, const int64_t *nn, ffts_layout &L )
+{
+    if ( layout == FFTM_MI_LAYOUT_3D )
+    {
+        L.dim = 3;
+        for ( int k = 0; k < 3; ++k )
+            L.n[k] = nn[k];
+        const int64_t nx = nn[0], ny = nn[1], nz = nn[2], nzh = nz / 2 + 1;
+        (void)nx;
+        L.sr[0] = ny * nz;
+        L.sr[1] = nz;
+        L.sr[2] = 1;
+        L.sc[0] = ny * nzh;
+        L.sc[1] = nzh;
+        L.sc[2] = 1;
+        return ok();
+    }
+    if ( layout < FFTM_MI_LAYOUT_4D_PENCIL || layout > FFTM_MI_LAYOUT_4D_XZWY )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi: unknown ffts layout" );
+    L.dim = 4;
+    for ( int k = 0; k < 4; ++k )
+        L.n[k] = nn[k];
+    const int64_t nx = nn[0], ny = nn[1], nz = nn[2], nw = nn[3], nwh = nw / 2 + 1;
+    if ( layout == FFTM_MI_LAYOUT_4D_PENCIL )
+    {
+        // real [w][z][y][x] (ffts.hpp:126, arranger 0123)
+        L.sr[0] = 1;
+        L.sr[1] = nx;
+        L.sr[2] = nx * ny;
+        L.sr[3] = nx * ny * nz;
+    }
+    else
+    {
+        // real [x][y][z][w] (ffts.hpp:144, arranger 3210)
+        L.sr[0] = ny * nz * nw;
+        L.sr[1] = nz * nw;
+        L.sr[2] = nw;
+        L.sr[3] = 1;
+    }
+    if ( layout == FFTM_MI_LAYOUT_4D_SLAB )
+    {
+        // spectrum [x][y][z][wh] (ffts.hpp:149-150, arranger 2103 over (y,z,w,x))
+        L.sc[0] = ny * nz * nwh;
+        L.sc[1] = nz * nwh;
+        L.sc[2] = nwh;
+        L.sc[3] = 1;
+    }
+    else
+    {
+        // spectrum [y][z][wh][x] (ffts.hpp:133-134, arranger 3210 over (y,z,w,x);
+        // fftm native xzwy, mpi_transpose_4d_same_xw.h:986-988)
+        L.sc[0] = 1;
+        L.sc[1] = nz * nwh * nx;
+        L.sc[2] = nwh * nx;
+        L.sc[3] = nx;
+    }
+    return ok();
+}
+
+int build_ffts_passes( const ffts_layout &L, bool overwrite, std::vector<pass_rt> &fwd, std::vector<pass_rt> &bwd,
+                       int64_t &work_elems )
+{
+    const int d  = L.dim;
+    const int R  = d - 1; // real axis (last logical)
+    int64_t   extc[4];
+    for ( int k = 0; k < d; ++k )
+        extc[k] = L.n[k];
+    extc[R] = L.n[R] / 2 + 1;
+    int rc  = 0;
+    fwd.clear();
+    bwd.clear();
+    // forward: real axis first (real -> spectrum layout), then C2C over axes R-1 .. 0 in place
+    fwd.push_back( axis_pass( KIND_R2C, false, R, d, L.n, L.sr, L.sc, BUF_IN, BUF_OUT, L.n[R], rc ) );
+    if ( rc )
+        return rc;
+    for ( int ax = R - 1; ax >= 0; --ax )
+    {
+        fwd.push_back( axis_pass( KIND_C2C, false, ax, d, extc, L.sc, L.sc, BUF_OUT, BUF_OUT, L.n[ax], rc ) );
+        if ( rc )
+            return rc;
+    }
+    // backward: C2C over axes 0 .. R-1 (first reads the input), then C2R into the real layout
+    const int scratch = overwrite ? BUF_IN : BUF_WORK;
+    work_elems        = overwrite ? 0 : extc[0] * extc[1] * extc[2] * ( d == 4 ? extc[3] : 1 );
+    for ( int ax = 0; ax < R; ++ax )
+    {
+        bwd.push_back(
+            axis_pass( KIND_C2C, true, ax, d, extc, L.sc, L.sc, ax == 0 ? BUF_IN : scratch, scratch, L.n[ax], rc ) );
+        if ( rc )
+            return rc;
+    }
+    bwd.push_back( axis_pass( KIND_C2R, true, R, d, L.n, L.sc, L.sr, R > 0 ? scratch : BUF_IN, BUF_OUT, L.n[R], rc ) );
+    return rc;
+}
+
+} // namespace
+} // namespace fftm_mi
+
+using namespace fftm_mi;
+
+struct fftm_mi_plan_s
+{
+    fftm_mi_plan_desc    desc{};
+    std::vector<pass_rt> passes;
+    tables               tb;
+    size_t               work_size = 0;
+    void                *work      = nullptr;
+    hipStream_t          stream    = nullptr;
+};
+
+struct fftm_mi_ffts_s
+{
+    int32_t              layout = 0;
+    int32_t              prec   = 0;
+    int32_t              flags  = 0;
+    int64_t              n[4]   = { 0, 0, 0, 0 };
+    std::vector<pass_rt> fwd, bwd;
+    tables               tb;
+    size_t               work_size = 0;
+    void                *work      = nullptr;
+    hipStream_t          stream    = nullptr;
+};
+
+extern "C" {
+
+const char *fftm_mi_last_error( void )
+{
+    return g_err.c_str();
+}
+
+int fftm_mi_version( void )
+{
+    return 100; // 0.1.0
+}
+
+int fftm_mi_plan_validate( const fftm_mi_plan_desc *desc, size_t *work_size )
+{
+    g_err.clear();
+    if ( !desc )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_plan_validate: null descriptor" );
+    std::vector<pass_rt> passes;
+    int64_t              we = 0;
+    int                  rc = build_plan_passes( *desc, passes, we );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    rc = prepare_passes( passes, desc->precision, nullptr );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    if ( work_size )
+        *work_size = (size_t)we * (size_t)csize_of( desc->precision );
+    return ok();
+}
+
+int fftm_mi_plan_create( const fftm_mi_plan_desc *desc, fftm_mi_plan *plan )
+{
+    g_err.clear();
+    if ( !desc || !plan )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_plan_create: null argument" );
+    *plan = nullptr;
+    std::unique_ptr<fftm_mi_plan_s> p( new ( std::nothrow ) fftm_mi_plan_s() );
+    if ( !p )
+        return fail( FFTM_MI_ALLOC_FAILED, "fftm_mi_plan_create: out of host memory" );
+    p->desc    = *desc;
+    p->tb.prec = desc->precision;
+    int64_t we = 0;
+    int     rc = build_plan_passes( *desc, p->passes, we );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    rc = prepare_passes( p->passes, desc->precision, &p->tb );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    p->work_size = (size_t)we * (size_t)csize_of( desc->precision );
+    *plan        = p.release();
+    return ok();
+}
+
+int fftm_mi_plan_get_work_size( fftm_mi_plan plan, size_t *bytes )
+{
+    if ( !plan || !bytes )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_get_work_size: null argument" );
+    *bytes = plan->work_size;
+    return ok();
+}
+
+int fftm_mi_plan_set_work_area( fftm_mi_plan plan, void *work_area )
+{
+    if ( !plan )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_set_work_area: null plan" );
+    plan->work = work_area;
+    return ok();
+}
+
+int fftm_mi_plan_set_stream( fftm_mi_plan plan, void *stream )
+{
+    if ( !plan )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_set_stream: null plan" );
+    plan->stream = static_cast<hipStream_t>( stream );
+    return ok();
+}
+
+int fftm_mi_plan_exec( fftm_mi_plan plan, int32_t dir, const void *in, void *out, int32_t sync )
+{
+    if ( !plan )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_exec: null plan" );
+    const int32_t pd   = plan->desc.dir;
+    bool          flip = false;
+    if ( dir != pd )
+    {
+        const bool c2c = ( pd == FFTM_MI_C2CF || pd == FFTM_MI_C2CB ) && ( dir == FFTM_MI_C2CF || dir == FFTM_MI_C2CB );
+        if ( !c2c )
+            return fail( FFTM_MI_INVALID_TYPE, "fftm_mi_plan_exec: incompatible execution direction" );
+        flip = true;
+    }
+    if ( !in || !out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_plan_exec: null data pointer" );
+    if ( ( pd == FFTM_MI_R2C || pd == FFTM_MI_C2R ) && in == out )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_plan_exec: in-place R2C/C2R is not supported" );
+    if ( plan->work_size > 0 && !plan->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_plan_exec: plan needs a work area (set_work_area)" );
+    return run_passes( plan->passes, plan->tb, plan->desc.precision, in, out, plan->work, plan->stream, flip,
+                       sync != 0, "plan_exec" );
+}
+
+int fftm_mi_plan_get_descriptor( fftm_mi_plan plan, fftm_mi_plan_desc *desc )
+{
+    if ( !plan || !desc )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_get_descriptor: null argument" );
+    *desc = plan->desc;
+    return ok();
+}
+
+int fftm_mi_plan_num_passes( fftm_mi_plan plan, int32_t *passes )
+{
+    if ( !plan || !passes )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_num_passes: null argument" );
+    *passes = (int32_t)plan->passes.size();
+    return ok();
+}
+
+int fftm_mi_plan_destroy( fftm_mi_plan plan )
+{
+    if ( !plan )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_plan_destroy: null plan" );
+    delete plan;
+    return ok();
+}
+
+int fftm_mi_transpose_4d( int32_t elem_bytes, const int64_t src_dims[4], const int32_t perm[4], const void *src,
+                          void *dst, void *stream, int32_t sync )
+{
+    if ( !src_dims || !perm || !src || !dst )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_transpose_4d: null argument" );
+    if ( elem_bytes != 4 && elem_bytes != 8 && elem_bytes != 16 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_transpose_4d: elem_bytes must be 4, 8 or 16" );
+    int seen = 0;
+    for ( int k = 0; k < 4; ++k )
+    {
+        if ( perm[k] < 0 || perm[k] > 3 || ( seen & ( 1 << perm[k] ) ) )
+            return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_transpose_4d: perm is not a permutation of 0..3" );
+        seen |= 1 << perm[k];
+        if ( src_dims[k] < 1 )
+            return fail( FFTM_MI_INVALID_SIZE, "fftm_mi_transpose_4d: dims must be >= 1" );
+    }
+    if ( src == dst )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_transpose_4d: in-place transpose is not supported" );
+    tr_params p;
+    p.src = src;
+    p.dst = dst;
+    int64_t dd[4], dstr[4];
+    for ( int k = 0; k < 4; ++k )
+    {
+        p.d[k] = src_dims[k];
+        dd[k]  = src_dims[perm[k]];
+    }
+    dstr[3] = 1;
+    for ( int k = 2; k >= 0; --k )
+        dstr[k] = dstr[k + 1] * dd[k + 1];
+    for ( int k = 0; k < 4; ++k )
+        p.ds[perm[k]] = dstr[k];
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    if ( perm[3] == 3 )
+    {
+        const int64_t rows = p.d[0] * p.d[1] * p.d[2];
+        const int64_t gx   = std::min<int64_t>( ( p.d[3] + 255 ) / 256, 1024 );
+        const int64_t gy   = std::min<int64_t>( rows, 65535 );
+        const int64_t gz   = ( rows + gy - 1 ) / gy;
+        const dim3    grid( (unsigned)gx, (unsigned)gy, (unsigned)gz );
+        if ( elem_bytes == 4 )
+            hipLaunchKernelGGL( k_tr_rows<float>, grid, dim3( 256 ), 0, s, p );
+        else if ( elem_bytes == 8 )
+            hipLaunchKernelGGL( k_tr_rows<float2>, grid, dim3( 256 ), 0, s, p );
+        else
+            hipLaunchKernelGGL( k_tr_rows<double2>, grid, dim3( 256 ), 0, s, p );
+    }
+    else
+    {
+        const int A = perm[3];
+        int       U = -1, V = -1;
+        for ( int k = 0; k < 3; ++k )
+            if ( k != A )
+            {
+                if ( U < 0 )
+                    U = k;
+                else
+                    V = k;
+            }
+        const int64_t gz = p.d[U] * p.d[V];
+        if ( gz > 65535 || ( p.d[A] + 31 ) / 32 > 65535 )
+            return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_transpose_4d: grid too large" );
+        const dim3 grid( (unsigned)( ( p.d[3] + 31 ) / 32 ), (unsigned)( ( p.d[A] + 31 ) / 32 ), (unsigned)gz );
+        if ( elem_bytes == 4 )
+            hipLaunchKernelGGL( k_tr_tile<float>, grid, dim3( 256 ), 0, s, p, A, U, V );
+        else if ( elem_bytes == 8 )
+            hipLaunchKernelGGL( k_tr_tile<float2>, grid, dim3( 256 ), 0, s, p, A, U, V );
+        else
+            hipLaunchKernelGGL( k_tr_tile<double2>, grid, dim3( 256 ), 0, s, p, A, U, V );
+    }
+    hipError_t e = hipGetLastError();
+    if ( e != hipSuccess )
+        return fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_transpose_4d: launch failed: " ) + hipGetErrorString( e ) );
+    if ( sync )
+    {
+        e = hipDeviceSynchronize();
+        if ( e != hipSuccess )
+            return fail( FFTM_MI_EXEC_FAILED,
+                         std::string( "fftm_mi_transpose_4d: synchronization failed: " ) + hipGetErrorString( e ) );
+    }
+    return ok();
+}
+
+int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, int32_t flags, size_t *work_size )
+{
+    g_err.clear();
+    if ( !n )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_validate: null sizes" );
+    if ( precision != FFTM_MI_F32 && precision != FFTM_MI_F64 )
+        return fail( FFTM_MI_INVALID_TYPE, "fftm_mi_ffts_validate: unknown precision" );
+    const int dim = layout == FFTM_MI_LAYOUT_3D ? 3 : 4;
+    for ( int k = 0; k < dim; ++k )
+        if ( n[k] < 1 )
+            return fail( FFTM_MI_INVALID_SIZE, "fftm_mi_ffts_validate: sizes must be >= 1" );
+    ffts_layout L;
+    int         rc = make_ffts_layout( layout, n, L );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    std::vector<pass_rt> f, b;
+    int64_t              we = 0;
+    rc = build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, f, b, we );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    rc = prepare_passes( f, precision, nullptr );
+    if ( rc == FFTM_MI_SUCCESS )
+        rc = prepare_passes( b, precision, nullptr );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    if ( work_size )
+        *work_size = (size_t)we * (size_t)csize_of( precision );
+    return ok();
+}
+
+int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, int32_t flags, fftm_mi_ffts *h )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_create: null handle" );
+    *h        = nullptr;
+    size_t ws = 0;
+    int    rc = fftm_mi_ffts_validate( layout, n, precision, flags, &ws );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    std::unique_ptr<fftm_mi_ffts_s> p( new ( std::nothrow ) fftm_mi_ffts_s() );
+    if ( !p )
+        return fail( FFTM_MI_ALLOC_FAILED, "fftm_mi_ffts_create: out of host memory" );
+    p->layout  = layout;
+    p->prec    = precision;
+    p->flags   = flags;
+    p->tb.prec = precision;
+    const int dim = layout == FFTM_MI_LAYOUT_3D ? 3 : 4;
+    for ( int k = 0; k < dim; ++k )
+        p->n[k] = n[k];
+    ffts_layout L;
+    make_ffts_layout( layout, n, L );
+    int64_t we = 0;
+    build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, p->fwd, p->bwd, we );
+    rc = prepare_passes( p->fwd, precision, &p->tb );
+    if ( rc == FFTM_MI_SUCCESS )
+        rc = prepare_passes( p->bwd, precision, &p->tb );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    p->work_size = ws;
+    *h           = p.release();
+    return ok();
+}
+
+int fftm_mi_ffts_get_work_size( fftm_mi_ffts h, size_t *bytes )
+{
+    if ( !h || !bytes )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_get_work_size: null argument" );
+    *bytes = h->work_size;
+    return ok();
+}
+
+int fftm_mi_ffts_set_work_area( fftm_mi_ffts h, void *work_area )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_set_work_area: null handle" );
+    h->work = work_area;
+    return ok();
+}
+
+int fftm_mi_ffts_set_stream( fftm_mi_ffts h, void *stream )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_set_stream: null handle" );
+    h->stream = static_cast<hipStream_t>( stream );
+    return ok();
+}
+
+int fftm_mi_ffts_forward( fftm_mi_ffts h, const void *in, void *out, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_forward: null handle" );
+    if ( !in || !out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_forward: null data pointer" );
+    if ( in == out )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_forward: in-place transform is not supported" );
+    return run_passes( h->fwd, h->tb, h->prec, in, out, h->work, h->stream, false, sync != 0, "ffts_forward" );
+}
+
+int fftm_mi_ffts_backward( fftm_mi_ffts h, const void *in, void *out, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_backward: null handle" );
+    if ( !in || !out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_backward: null data pointer" );
+    if ( in == out )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_backward: in-place transform is not supported" );
+    if ( h->work_size > 0 && !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_backward: a work area is required (set_work_area)" );
+    return run_passes( h->bwd, h->tb, h->prec, in, out, h->work, h->stream, false, sync != 0, "ffts_backward" );
+}
+
+int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *backward_passes )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_num_passes: null handle" );
+    if ( forward_passes )
+        *forward_passes = (int32_t)h->fwd.size();
+    if ( backward_passes )
+        *backward_passes = (int32_t)h->bwd.size();
+    return ok();
+}
+
+int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_run_pass: null handle" );
+    const std::vector<pass_rt> &v = backward ? h->bwd : h->fwd;
+    if ( index < 0 || index >= (int32_t)v.size() )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_run_pass: pass index out of range" );
+    if ( !in || !out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_run_pass: null data pointer" );
+    if ( backward && h->work_size > 0 && !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_run_pass: a work area is required (set_work_area)" );
+    std::vector<pass_rt> one( 1, v[(size_t)index] );
+    return run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, sync != 0, "ffts_run_pass" );
+}
+
+int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, int64_t *bytes )
+{
+    if ( !h || !bytes )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_pass_bytes: null argument" );
+    const std::vector<pass_rt> &v = backward ? h->bwd : h->fwd;
+    if ( index < 0 || index >= (int32_t)v.size() )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_pass_bytes: pass index out of range" );
+    const pass_rt &p  = v[(size_t)index];
+    const int64_t  cs = csize_of( h->prec ), rs = cs / 2;
+    const int64_t  nh = p.N / 2 + 1;
+    if ( p.kind == KIND_C2C )
+        *bytes = 2 * p.ncols * p.N * cs;
+    else
+        *bytes = p.ncols * ( p.N * rs + nh * cs );
+    return ok();
+}
+
+int fftm_mi_ffts_destroy( fftm_mi_ffts h )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_destroy: null handle" );
+    delete h;
+    return ok();
+}
+
+} // extern "C"

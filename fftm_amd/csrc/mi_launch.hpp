@@ -1,0 +1,30 @@
+// Host-side launch entry points for the pass kernels (one translation unit per
+// precision instantiates the compile-time-N kernels; see mi_kernels_f32.hip /
+// mi_kernels_f64.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mi_pass.hpp"
+
+namespace fftm_mi
+{
+
+// compiled lengths of the fixed-N kernels: powers of two 2..4096
+inline bool fixed_supported( int64_t N )
+{
+    return N >= 2 && N <= 4096 && ( N & ( N - 1 ) ) == 0;
+}
+
+// columns per workgroup of a fixed-N kernel (complex lines)
+int fixed_tile_cols_f32( int N, bool loadb );
+int fixed_tile_cols_f64( int N, bool loadb );
+
+hipError_t launch_pass_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, hipStream_t s );
+hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool inv, bool loadb, hipStream_t s );
+hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
+                               hipStream_t s );
+hipError_t launch_generic_f64( const pass_params<double> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
+                               hipStream_t s );
+
+} // namespace fftm_mi

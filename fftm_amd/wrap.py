@@ -1,0 +1,404 @@
+"""Host mirror of fftm's backend-neutral ``external_wrap`` boundary over the C ABI.
+
+* ``fft``            -- one plan: the surface of ``fftm::wrap::fft_base<T>`` /
+                         ``hipfft::fft<T,D>`` (source/external_wrap/hipfft_wrap.h:472-496, 499-1240).
+* ``fft_wrap_many``  -- the named-plan container ``fft_wrap_many<BaseFFTWrap,T>``,
+                         subset used by ``ffts`` (source/external_wrap/fft_wrap_many.h:120-292,
+                         375-383, 1606-1680, 1753-1802).
+* ``runtime_api``    -- the members of ``hip_runtime_api`` that ffts uses
+                         (hipfft_wrap.h:42-257): device_synchronize, stream creation, events.
+
+Device memory is whatever the caller passes: a CUDA/HIP ``torch.Tensor`` or a raw
+integer device address.  Error behaviour follows the reference: API misuse raises
+``LogicError`` (std::logic_error), unknown plan names raise ``KeyError``
+(std::map::at -> std::out_of_range), backend failures raise ``FftmMiError``
+(std::runtime_error from FFTM_HIPFFT_SAFE_CALL).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+from . import _lib
+
+
+class LogicError(RuntimeError):
+    """API misuse (the reference throws std::logic_error)."""
+
+
+class direction(enum.IntEnum):
+    """fftm::direction (source/fft_direction.h:9-15)."""
+    R2C = 1
+    C2R = 2
+    C2CF = 3
+    C2CB = 4
+
+
+def fft_direction_name(d: direction) -> str:
+    return {direction.R2C: "R2C", direction.C2R: "C2R", direction.C2CF: "C2CF",
+            direction.C2CB: "C2CB"}.get(direction(d), "unknown")
+
+
+@dataclass
+class fft_plan_descriptor:
+    """fftm::wrap::fft_plan_descriptor (source/external_wrap/fft_plan_descriptor.h:15-32)."""
+    dir: direction = direction.R2C
+    backend_type: int = 0
+    rank: int = 0
+    default_layout: bool = False
+    n: Tuple[int, int, int] = (0, 0, 0)
+    inembed: Tuple[int, int, int] = (0, 0, 0)
+    istride: int = 0
+    idist: int = 0
+    onembed: Tuple[int, int, int] = (0, 0, 0)
+    ostride: int = 0
+    odist: int = 0
+    batch: int = 0
+    work_size: int = 0
+    work_area_token: int = 0
+    stream_token: int = 0
+
+
+def _precision_of(real) -> int:
+    import torch
+    if real in (torch.float32, "float32", "float", "f32"):
+        return _lib.F32
+    if real in (torch.float64, "float64", "double", "f64"):
+        return _lib.F64
+    raise TypeError(f"unsupported real type {real!r} (float32 / float64)")
+
+
+def device_ptr(x) -> int:
+    """Raw device address of a tensor (or pass an int through)."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_cuda:
+            raise ValueError("fftm_amd: tensor must live on the GPU")
+        return int(x.data_ptr())
+    raise TypeError(f"cannot take a device address of {type(x).__name__}")
+
+
+def stream_handle(stream) -> int:
+    if stream is None:
+        return 0
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+class runtime_api:
+    """Members of hip_runtime_api (hipfft_wrap.h:42-257) used around ffts."""
+
+    @staticmethod
+    def device_synchronize():
+        import torch
+        torch.cuda.synchronize()
+
+    @staticmethod
+    def create_nonblocking_stream():
+        import torch
+        return torch.cuda.Stream()
+
+    @staticmethod
+    def stream_synchronize(stream):
+        stream.synchronize()
+
+    @staticmethod
+    def stream_ready(stream) -> bool:
+        return bool(stream.query())
+
+
+class fft:
+    """One plan -- fft<T,D> (hipfft_wrap.h:499-1240) on the MI355X backend.
+
+    ``fft(real, D, n, inembed, istride, idist, onembed, ostride, odist, batch)``
+    for the advanced layout (constructors :520-580), or ``fft(real, D, n, batch=b)``
+    for the default layout (:582-602).  ``n``/``inembed``/``onembed`` are sequences
+    of length rank (1..3)."""
+
+    def __init__(self, real, D, n: Sequence[int], inembed: Optional[Sequence[int]] = None,
+                 istride: int = 1, idist: int = 0, onembed: Optional[Sequence[int]] = None,
+                 ostride: int = 1, odist: int = 0, batch: int = 1, flags: int = 0):
+        self._handle = None
+        L = _lib.lib()
+        D = direction(D)
+        rank = len(n)
+        if not 1 <= rank <= 3:
+            raise LogicError("fft: supports only 1D, 2D, and 3D plans")
+        desc = _lib.PlanDesc()
+        desc.dir = int(D)
+        desc.precision = _precision_of(real)
+        desc.rank = rank
+        desc.default_layout = 1 if inembed is None else 0
+        for k in range(rank):
+            desc.n[k] = int(n[k])
+            if inembed is not None:
+                desc.inembed[k] = int(inembed[k])
+                desc.onembed[k] = int(onembed[k])
+        desc.istride, desc.idist = int(istride), int(idist)
+        desc.ostride, desc.odist = int(ostride), int(odist)
+        desc.batch = int(batch)
+        desc.flags = int(flags)
+        h = ctypes.c_void_p()
+        _lib.check(L.fftm_mi_plan_create(ctypes.byref(desc), ctypes.byref(h)), "fftm_mi_plan_create")
+        self._handle = h
+        self._desc = desc
+        self._dir = D
+        self._real = real
+        self._work_token = 0
+        self._stream_token = 0
+        ws = ctypes.c_size_t()
+        _lib.check(L.fftm_mi_plan_get_work_size(h, ctypes.byref(ws)), "fftm_mi_plan_get_work_size")
+        self.work_size = int(ws.value)
+
+    # -- fft_base<T> surface ------------------------------------------------
+    def exec(self, inp, out):
+        self._exec(self._dir, inp, out, sync=True)
+
+    def exec_no_sync(self, inp, out):
+        self._exec(self._dir, inp, out, sync=False)
+
+    def exec_direction(self, exec_dir, inp, out):
+        self._exec(direction(exec_dir), inp, out, sync=True)
+
+    def exec_direction_no_sync(self, exec_dir, inp, out):
+        self._exec(direction(exec_dir), inp, out, sync=False)
+
+    def _exec(self, d, inp, out, sync):
+        self._check_alive()
+        if d != self._dir and not (self._dir in (direction.C2CF, direction.C2CB)
+                                   and d in (direction.C2CF, direction.C2CB)):
+            raise LogicError("fft::exec_typed_direction: incompatible execution direction")
+        _lib.check(_lib.lib().fftm_mi_plan_exec(self._handle, int(d), device_ptr(inp), device_ptr(out),
+                                                1 if sync else 0), "fftm_mi_plan_exec")
+
+    def get_direction(self) -> direction:
+        return self._dir
+
+    def descriptor(self) -> fft_plan_descriptor:
+        d = self._desc
+        r = d.rank
+        pad = lambda a: tuple(int(a[k]) if k < r else 0 for k in range(3))
+        return fft_plan_descriptor(
+            dir=self._dir, backend_type=int(self._dir), rank=r, default_layout=bool(d.default_layout),
+            n=pad(d.n), inembed=pad(d.inembed), istride=int(d.istride), idist=int(d.idist),
+            onembed=pad(d.onembed), ostride=int(d.ostride), odist=int(d.odist), batch=int(d.batch),
+            work_size=self.work_size, work_area_token=self._work_token, stream_token=self._stream_token)
+
+    def get_work_size(self) -> int:
+        return self.work_size
+
+    def set_work_area(self, work_area):
+        self._check_alive()
+        ptr = device_ptr(work_area)
+        self._work_token = ptr
+        self._work_ref = work_area
+        _lib.check(_lib.lib().fftm_mi_plan_set_work_area(self._handle, ptr), "fftm_mi_plan_set_work_area")
+
+    def set_stream(self, stream):
+        self._check_alive()
+        h = stream_handle(stream)
+        self._stream_token = h
+        _lib.check(_lib.lib().fftm_mi_plan_set_stream(self._handle, h), "fftm_mi_plan_set_stream")
+
+    def recreate_with_stream_and_work_area(self, stream, work_area):
+        # plans hold no state beyond the stream and the work area
+        self.set_stream(stream)
+        self.set_work_area(work_area)
+
+    def num_passes(self) -> int:
+        n = ctypes.c_int32()
+        _lib.check(_lib.lib().fftm_mi_plan_num_passes(self._handle, ctypes.byref(n)), "fftm_mi_plan_num_passes")
+        return int(n.value)
+
+    def destroy(self):
+        if self._handle is not None:
+            _lib.lib().fftm_mi_plan_destroy(self._handle)
+            self._handle = None
+
+    def _check_alive(self):
+        if self._handle is None:
+            raise LogicError("fft: plan has been destroyed")
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class fft_wrap_many:
+    """Named-plan container (fft_wrap_many.h:25-1927), the subset ffts uses.
+
+    The fftm-only plan sequences, C2C plan arrays and R2C/C2R bundles
+    (fft_wrap_many.h:188-237, 453-1605) belong to the MPI frontend and raise
+    ``NotImplementedError``."""
+
+    def __init__(self, real):
+        self.real = real
+        self._plans: Dict[str, fft] = {}
+        self._work_area_size = 0
+        self._activated = False
+        self._external_activated = False
+        self._hot_exec_no_sync = False
+        self._work = None
+        self._memory_profiler = None
+        self._memory_profile_prefix = ""
+
+    # -- plan creation (fft_wrap_many.h:120-186) ----------------------------
+    def add_plan(self, D, name: str, n, inembed, istride, idist, onembed, ostride, odist, batch):
+        if self._activated:
+            raise LogicError("fft_wrap_many::add_plan: cannot add new plans after activation.")
+        if name in self._plans:
+            return  # std::map::emplace keeps the existing entry
+        self._plans[name] = fft(self.real, D, n, inembed, istride, idist, onembed, ostride, odist, batch)
+
+    def add_plan_1D(self, D, name, n, inembed, istride, idist, onembed, ostride, odist, batch):
+        self.add_plan(D, name, [n], [inembed], istride, idist, [onembed], ostride, odist, batch)
+
+    def add_plan_1D_default(self, D, name, n, batch):
+        if self._activated:
+            raise LogicError("fft_wrap_many::add_plan_1D_default: cannot add new plans after activation.")
+        if name not in self._plans:
+            self._plans[name] = fft(self.real, D, [n], batch=batch)
+
+    def add_plan_2D(self, D, name, n0, n1, inembed0, inembed1, istride, idist, onembed0, onembed1, ostride,
+                    odist, batch):
+        self.add_plan(D, name, [n0, n1], [inembed0, inembed1], istride, idist, [onembed0, onembed1], ostride,
+                      odist, batch)
+
+    def add_plan_3D(self, D, name, n0, n1, n2, inembed0, inembed1, inembed2, istride, idist, onembed0,
+                    onembed1, onembed2, ostride, odist, batch):
+        self.add_plan(D, name, [n0, n1, n2], [inembed0, inembed1, inembed2], istride, idist,
+                      [onembed0, onembed1, onembed2], ostride, odist, batch)
+
+    # -- work area (fft_wrap_many.h:239-292) ---------------------------------
+    def activate(self):
+        if self._activated:
+            raise LogicError("fft_wrap_many::activate: cannot activate again.")
+        if self._external_activated:
+            raise LogicError("fft_wrap_many::activate: cannot use external and internal activation, the class "
+                             "is externally activated.")
+        self.activate_work_size()
+        if self._work_area_size > 0:
+            import torch
+            self._work = torch.empty(self._work_area_size, dtype=torch.uint8, device="cuda")
+            for p in self._plans.values():
+                p.set_work_area(self._work)
+        self._update_memory_profile()
+        self._activated = True
+
+    def activate_work_size(self) -> int:
+        for p in self._plans.values():
+            self._work_area_size = max(self._work_area_size, p.get_work_size())
+        return self._work_area_size
+
+    def set_external_work_area(self, work_area):
+        if self._activated:
+            raise LogicError("fft_wrap_many::set_external_work_area: cannot set external work_area with activated "
+                             "local work_area.")
+        for p in self._plans.values():
+            p.set_work_area(work_area)
+        self._external_activated = True
+        self._update_memory_profile()
+
+    def get_work_size(self) -> int:
+        return self._work_area_size
+
+    def set_memory_profiler(self, profiler, prefix: str):
+        self._memory_profiler = profiler
+        self._memory_profile_prefix = prefix
+        self._update_memory_profile()
+
+    def set_hot_exec_no_sync_enabled(self, enabled: bool):
+        self._hot_exec_no_sync = bool(enabled)
+
+    # -- execution (fft_wrap_many.h:375-395) ---------------------------------
+    def exec(self, name: str, inp, out):
+        plan = self._plans[name]
+        if self._hot_exec_no_sync:
+            plan.exec_no_sync(inp, out)
+        else:
+            plan.exec(inp, out)
+
+    def exec_named_no_sync(self, name: str, inp, out):
+        self._plans[name].exec_no_sync(inp, out)
+
+    def synchronize_device_execution(self):
+        runtime_api.device_synchronize()
+
+    # -- per-plan accessors (fft_wrap_many.h:1628-1680) ----------------------
+    def work_size(self, name: str) -> int:
+        return self._plans[name].get_work_size()
+
+    def set_work_area(self, name: str, work_area):
+        self._plans[name].set_work_area(work_area)
+
+    def set_stream(self, name: str, stream):
+        self._plans[name].set_stream(stream)
+
+    def recreate_plan_with_stream_and_work_area(self, name: str, stream, work_area):
+        self._plans[name].recreate_with_stream_and_work_area(stream, work_area)
+
+    def plan_descriptor(self, name: str) -> fft_plan_descriptor:
+        return self._plans[name].descriptor()
+
+    def plan_descriptors(self) -> List[Tuple[str, fft_plan_descriptor]]:
+        return [(k, self._plans[k].descriptor()) for k in sorted(self._plans)]
+
+    def plan(self, name: str) -> fft:
+        return self._plans[name]
+
+    # -- lifecycle (fft_wrap_many.h:1753-1802) -------------------------------
+    def release(self):
+        for p in self._plans.values():
+            p.destroy()
+        self._plans.clear()
+        self._work = None
+        self._work_area_size = 0
+        self._activated = False
+        self._external_activated = False
+        self._hot_exec_no_sync = False
+        self._update_memory_profile()
+
+    def _update_memory_profile(self):
+        if self._memory_profiler is None or not self._memory_profile_prefix:
+            return
+        self._memory_profiler.set_bytes(self._memory_profile_prefix + "/work_area",
+                                        0 if self._external_activated else self._work_area_size)
+
+    # -- fftm-only surface ---------------------------------------------------
+    def _fftm_only(self, *a, **k):
+        raise NotImplementedError("fft_wrap_many: plan sequences, C2C plan arrays and R2C/C2R bundles belong to "
+                                  "the MPI fftm frontend and are not supported by this backend")
+
+    make_r2c_c2r_plan_bundle_2D = _fftm_only
+    make_plan_sequence = _fftm_only
+    make_c2c_plan_array = _fftm_only
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def transpose_4d(src, dst, perm: Sequence[int], stream=None, sync: bool = False):
+    """dst = transpose(src, perm) for 4D memory arrays (torch tensors, contiguous)."""
+    if src.dim() != 4:
+        raise ValueError("transpose_4d: src must be a 4D tensor")
+    exp = tuple(src.shape[p] for p in perm)
+    if tuple(dst.shape) != exp:
+        raise ValueError(f"transpose_4d: destination shape {tuple(dst.shape)} != expected {exp}")
+    if not (src.is_contiguous() and dst.is_contiguous()) or src.dtype != dst.dtype:
+        raise ValueError("transpose_4d: tensors must be contiguous and of one dtype")
+    L = _lib.lib()
+    _lib.check(L.fftm_mi_transpose_4d(src.element_size(), _lib.i64_array(src.shape), _lib.i32_array(perm),
+                                      device_ptr(src), device_ptr(dst), stream_handle(stream), 1 if sync else 0),
+               "fftm_mi_transpose_4d")

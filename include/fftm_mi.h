@@ -1,0 +1,182 @@
+/*
+ * fftm_mi.h -- C ABI of the MI355X-native FFT backend for fftm's single-GPU `ffts`
+ * R2C/C2R path.
+ *
+ * The library replaces the hipFFT calls behind fftm's backend-neutral
+ * `external_wrap` boundary.  Every entry point takes plain device pointers, sizes
+ * and an opaque HIP stream (`void *`, a hipStream_t; NULL = the null stream).  No
+ * C++ or torch types cross this boundary.
+ *
+ * Status codes use hipfftResult's numbering so that a caller's existing
+ * hipfft_error_string / safe-call logic keeps working
+ * (source/external_wrap/hipfft_safe_call.h:16-54).  On failure
+ * fftm_mi_last_error() returns a message for the calling thread.
+ *
+ * Synchronisation mirrors the reference: with sync != 0 an exec ends with
+ * hipDeviceSynchronize() and reports asynchronous errors
+ * (FFTM_HIPFFT_SAFE_CALL, hipfft_safe_call.h:58-75); with sync == 0 it only checks
+ * the launch (hipfft_check_no_sync, hipfft_wrap.h:31-40).
+ */
+#ifndef FFTM_MI_H
+#define FFTM_MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- status codes (hipfftResult numbering) -------------------------------- */
+#define FFTM_MI_SUCCESS 0
+#define FFTM_MI_INVALID_PLAN 1
+#define FFTM_MI_ALLOC_FAILED 2
+#define FFTM_MI_INVALID_TYPE 3
+#define FFTM_MI_INVALID_VALUE 4
+#define FFTM_MI_INTERNAL_ERROR 5
+#define FFTM_MI_EXEC_FAILED 6
+#define FFTM_MI_SETUP_FAILED 7
+#define FFTM_MI_INVALID_SIZE 8
+#define FFTM_MI_UNALIGNED_DATA 9
+#define FFTM_MI_INVALID_DEVICE 11
+#define FFTM_MI_NO_WORKSPACE 13
+#define FFTM_MI_NOT_IMPLEMENTED 14
+#define FFTM_MI_NOT_SUPPORTED 16
+
+/* --- directions: same values as fftm::direction (source/fft_direction.h:9-15) */
+#define FFTM_MI_R2C 1
+#define FFTM_MI_C2R 2
+#define FFTM_MI_C2CF 3
+#define FFTM_MI_C2CB 4
+
+/* --- precisions ----------------------------------------------------------- */
+#define FFTM_MI_F32 0 /* float real / float2 complex (hipfftReal / hipfftComplex)          */
+#define FFTM_MI_F64 1 /* double real / double2 complex (hipfftDoubleReal / ...Complex)     */
+
+/* --- plan flags ----------------------------------------------------------- */
+/* C2R of rank >= 2 keeps its input intact by default and needs a work area of one
+ * input-sized complex buffer.  With this flag the input is used as scratch
+ * (hipFFT's documented out-of-place C2R behaviour) and the work size is 0. */
+#define FFTM_MI_FLAG_OVERWRITE_INPUT 1
+
+/*
+ * Plan descriptor: the fields of fftm::wrap::fft_plan_descriptor
+ * (source/external_wrap/fft_plan_descriptor.h:15-32) that define a transform.
+ * default_layout != 0 means tight row-major layout (inembed/onembed ignored),
+ * as the fft(n, batch) constructor does (hipfft_wrap.h:582-602).
+ */
+typedef struct fftm_mi_plan_desc
+{
+    int32_t dir;            /* FFTM_MI_R2C / _C2R / _C2CF / _C2CB             */
+    int32_t precision;      /* FFTM_MI_F32 / FFTM_MI_F64                       */
+    int32_t rank;           /* 1..3                                            */
+    int32_t default_layout; /* 0: use inembed/istride/idist/onembed/...       */
+    int64_t n[3];
+    int64_t inembed[3];
+    int64_t istride;
+    int64_t idist;
+    int64_t onembed[3];
+    int64_t ostride;
+    int64_t odist;
+    int64_t batch;
+    int32_t flags;          /* FFTM_MI_FLAG_*                                   */
+    int32_t reserved;
+} fftm_mi_plan_desc;
+
+typedef struct fftm_mi_plan_s *fftm_mi_plan;
+
+/* Host-only validation of a descriptor and its work size (no GPU call).
+ * Replaces the argument checks hipfftMakePlanMany64 performs
+ * (hipfft_wrap.h:1263-1266). */
+int fftm_mi_plan_validate( const fftm_mi_plan_desc *desc, size_t *work_size );
+
+/* Create a plan: hipfftCreate + hipfftSetAutoAllocation(0) + hipfftMakePlanMany64
+ * (source/external_wrap/hipfft_wrap.h:1253-1275, called from the fft<T,D>
+ * constructors :520-605).  Uploads the plan's twiddle tables to the current device. */
+int fftm_mi_plan_create( const fftm_mi_plan_desc *desc, fftm_mi_plan *plan );
+
+/* Work size in bytes: fft<T,D>::get_work_size (hipfft_wrap.h:608-611). */
+int fftm_mi_plan_get_work_size( fftm_mi_plan plan, size_t *bytes );
+
+/* Bind an external work area: hipfftSetWorkArea (hipfft_wrap.h:614-618). */
+int fftm_mi_plan_set_work_area( fftm_mi_plan plan, void *work_area );
+
+/* Bind a stream: hipfftSetStream (hipfft_wrap.h:620-624). */
+int fftm_mi_plan_set_stream( fftm_mi_plan plan, void *stream );
+
+/* Execute: hipfftExecR2C/D2Z/C2R/Z2D/C2C/Z2Z (hipfft_wrap.h:303-417) through
+ * fft<T,D>::exec / exec_no_sync / exec_direction(_no_sync) (:665-734).
+ * dir must equal the plan's direction, except that a C2C plan runs either C2C
+ * direction (exec_typed_direction, :686-712).  sync selects the safe-call
+ * device synchronisation. */
+int fftm_mi_plan_exec( fftm_mi_plan plan, int32_t dir, const void *in, void *out, int32_t sync );
+
+/* The descriptor the plan was created with: fft<T,D>::descriptor (:1226-1229). */
+int fftm_mi_plan_get_descriptor( fftm_mi_plan plan, fftm_mi_plan_desc *desc );
+
+/* Number of HBM passes (kernel launches) one exec runs (observability). */
+int fftm_mi_plan_num_passes( fftm_mi_plan plan, int32_t *passes );
+
+/* Destroy: hipfftDestroy (fft<T,D>::~fft, hipfft_wrap.h:600-606). */
+int fftm_mi_plan_destroy( fftm_mi_plan plan );
+
+/*
+ * 4D axis permutation dst = transpose(src, perm) of memory arrays (dims listed
+ * slowest..fastest; dst axis k is src axis perm[k]); elem_bytes in {4, 8, 16}.
+ * Replaces direct_transpose_4d (source/detail/direct_transpose_4d.h:42-120) and
+ * runtime_memcpy_4d_slab_transposer (source/detail/runtime_memcpy_4d_slab_transposer.h:24-198).
+ */
+int fftm_mi_transpose_4d( int32_t elem_bytes, const int64_t src_dims[4], const int32_t perm[4], const void *src,
+                          void *dst, void *stream, int32_t sync );
+
+/*
+ * Fused ffts transforms (source/ffts.hpp:356-383, 809-867): the whole forward or
+ * backward transform of one ffts instance, with the 4D stage transposes folded
+ * into the FFT passes (one HBM read + write per transformed axis).
+ *
+ *   layout FFTM_MI_LAYOUT_3D           real [x][y][z]    <-> spectrum [x][y][z/2+1]
+ *   layout FFTM_MI_LAYOUT_4D_PENCIL    real [w][z][y][x] <-> spectrum [y][z][w/2+1][x]
+ *   layout FFTM_MI_LAYOUT_4D_SLAB      real [x][y][z][w] <-> spectrum [x][y][z][w/2+1]
+ *   layout FFTM_MI_LAYOUT_4D_XZWY      real [x][y][z][w] <-> spectrum [y][z][w/2+1][x]
+ *     (the one-rank fftm native_xzwy spectrum, fftm.hpp:2941-2961, 4519-4560)
+ *
+ * n = {nx, ny, nz[, nw]}.  backward is unnormalised (x N) and keeps its input when a
+ * work area of fftm_mi_ffts_get_work_size bytes is bound; with
+ * FFTM_MI_FLAG_OVERWRITE_INPUT it uses the input as scratch (ffts 4D backward is
+ * destructive, ffts.hpp:377) and needs no work area.
+ */
+#define FFTM_MI_LAYOUT_3D 0
+#define FFTM_MI_LAYOUT_4D_PENCIL 1
+#define FFTM_MI_LAYOUT_4D_SLAB 2
+#define FFTM_MI_LAYOUT_4D_XZWY 3
+
+typedef struct fftm_mi_ffts_s *fftm_mi_ffts;
+
+int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, int32_t flags, size_t *work_size );
+int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, int32_t flags, fftm_mi_ffts *h );
+int fftm_mi_ffts_get_work_size( fftm_mi_ffts h, size_t *bytes );
+int fftm_mi_ffts_set_work_area( fftm_mi_ffts h, void *work_area );
+int fftm_mi_ffts_set_stream( fftm_mi_ffts h, void *stream );
+int fftm_mi_ffts_forward( fftm_mi_ffts h, const void *in, void *out, int32_t sync );
+int fftm_mi_ffts_backward( fftm_mi_ffts h, const void *in, void *out, int32_t sync );
+int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *backward_passes );
+/* Run one pass (0-based) of the forward (backward = 0) or backward (backward = 1)
+ * transform alone, on the handle's stream.  The same buffers as the full call are
+ * passed; the pass reads and writes exactly what it does inside that call.  Used to
+ * time individual kernels; it has no counterpart in the reference. */
+int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
+/* Algorithmic HBM bytes one pass moves (reads + writes of its input and output). */
+int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, int64_t *bytes );
+int fftm_mi_ffts_destroy( fftm_mi_ffts h );
+
+/* Last error message of the calling thread ("" when none). */
+const char *fftm_mi_last_error( void );
+
+/* Library version: major * 10000 + minor * 100 + patch. */
+int fftm_mi_version( void );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTM_MI_H */
