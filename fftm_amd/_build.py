@@ -22,7 +22,7 @@ SOURCES = ["mi_kernels_f32.hip", "mi_kernels_f64.hip", "mi_host.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
 ]
 
 
@@ -55,7 +55,24 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_tests(verbose)
     return LIB
+
+
+CPP_TESTS = [os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_ffts_cpp.cpp")]
+
+
+def build_cpp_tests(verbose: bool = False):
+    """The C++ drop-in test programs (host C++ over include/fftm_mi/*.hpp, linked to the library)."""
+    for src in CPP_TESTS:
+        out = os.path.splitext(src)[0] + ".bin"
+        rel = os.path.relpath(HERE, os.path.dirname(src))
+        cmd = [hipcc(), *HIPFLAGS, "-o", out, src, f"-L{HERE}", "-lfftm_mi", f"-Wl,-rpath,$ORIGIN/{rel}"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {out}")
 
 
 if __name__ == "__main__":
