@@ -93,6 +93,10 @@ SIGNATURES = {
     "fftm_mi_ffts_run_pass": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _i32]),
     "fftm_mi_ffts_pass_bytes": (ctypes.c_int, [_vp, _i32, _i32, _i64p]),
     "fftm_mi_ffts_destroy": (ctypes.c_int, [_vp]),
+    "fftm_mi_ffts_r2c_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
+    "fftm_mi_ffts_c2r_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
+    "fftm_mi_ffts_spectral_multiply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32]),
+    "fftm_mi_ffts_shell_gradient": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32]),
     "fftm_mi_last_error": (ctypes.c_char_p, []),
     "fftm_mi_version": (ctypes.c_int, []),
 }

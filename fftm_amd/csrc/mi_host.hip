@@ -15,6 +15,7 @@
 
 #include "../../include/fftm_mi.h"
 #include "mi_launch.hpp"
+#include "mi_spectral.hpp"
 #include "mi_transpose.hpp"
 
 namespace fftm_mi
@@ -62,6 +63,7 @@ struct pass_rt
     bool       generic = false;
     radix_list rl{};
     int        gen_cols = 1;
+    int        wmode    = 0; // real-axis bin weights (adjoints), see pass_params::wmode
     double     scale    = 1.0;
 };
 
@@ -247,7 +249,7 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.os1     = r.os1;
     p.os2     = r.os2;
     p.store_b = r.storeb ? 1 : 0;
-    p.pad_    = 0;
+    p.wmode   = r.wmode;
     p.scale   = (T)r.scale;
     if constexpr ( std::is_same<T, float>::value )
         return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
@@ -258,13 +260,15 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
 }
 
 int run_passes( const std::vector<pass_rt> &passes, const tables &tb, int prec, const void *in, void *out,
-                void *work, hipStream_t s, bool flip_c2c, bool sync, const char *what )
+                void *work, hipStream_t s, bool flip_c2c, bool sync, const char *what, int real_wmode = 0 )
 {
     for ( const pass_rt &r0 : passes )
     {
         pass_rt r = r0;
         if ( flip_c2c && r.kind == KIND_C2C )
             r.inv = !r.inv;
+        if ( r.kind != KIND_C2C )
+            r.wmode = real_wmode;
         const void *src = r.src == BUF_IN ? in : ( r.src == BUF_OUT ? (const void *)out : (const void *)work );
         void       *dst = r.dst == BUF_OUT ? out : ( r.dst == BUF_WORK ? work : const_cast<void *>( in ) );
         hipError_t  e   = prec == FFTM_MI_F32 ? launch_one<float>( r, tb, src, dst, s )
@@ -666,7 +670,44 @@ struct fftm_mi_ffts_s
     size_t               work_size = 0;
     void                *work      = nullptr;
     hipStream_t          stream    = nullptr;
+    spec_geom            sg{};
+    double              *partials  = nullptr; // shell-gradient partial sums (SHELL_BLOCKS x MAX_SHELLS)
+
+    ~fftm_mi_ffts_s()
+    {
+        if ( partials )
+            (void)hipFree( partials );
+    }
 };
+
+namespace
+{
+constexpr int SHELL_BLOCKS = 512;
+constexpr int MAX_SHELLS   = 1024;
+
+spec_geom make_spec_geom( const ffts_layout &L )
+{
+    spec_geom g{};
+    g.nd        = L.dim;
+    g.half_axis = L.dim - 1;
+    int order[4] = { 0, 1, 2, 3 };
+    std::sort( order, order + L.dim, [&]( int a, int b ) { return L.sc[a] > L.sc[b]; } );
+    g.total     = 1;
+    double kmax2 = 0.0;
+    for ( int m = 0; m < L.dim; ++m )
+    {
+        const int a = order[m];
+        g.ax[m]     = a;
+        g.ext[m]    = a == L.dim - 1 ? L.n[a] / 2 + 1 : L.n[a];
+        g.total *= g.ext[m];
+        g.n[a] = L.n[a];
+        kmax2 += (double)( L.n[a] / 2 ) * (double)( L.n[a] / 2 );
+    }
+    g.kscale  = kmax2 > 0 ? 1.0 / std::sqrt( kmax2 ) : 0.0; // times nshells at use
+    g.nshells = 1;
+    return g;
+}
+} // namespace
 
 extern "C" {
 
@@ -934,6 +975,7 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
     if ( rc != FFTM_MI_SUCCESS )
         return rc;
     p->work_size = ws;
+    p->sg        = make_spec_geom( L );
     *h           = p.release();
     return ok();
 }
@@ -1027,6 +1069,110 @@ int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, in
     else
         *bytes = p.ncols * ( p.N * rs + nh * cs );
     return ok();
+}
+
+int fftm_mi_ffts_r2c_adjoint( fftm_mi_ffts h, const void *spectrum_in, void *real_out, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_r2c_adjoint: null handle" );
+    if ( !spectrum_in || !real_out || spectrum_in == real_out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_r2c_adjoint: bad data pointers" );
+    if ( h->work_size > 0 && !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_r2c_adjoint: a work area is required (set_work_area)" );
+    return run_passes( h->bwd, h->tb, h->prec, spectrum_in, real_out, h->work, h->stream, false, sync != 0,
+                       "ffts_r2c_adjoint", 2 );
+}
+
+int fftm_mi_ffts_c2r_adjoint( fftm_mi_ffts h, const void *real_in, void *spectrum_out, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_c2r_adjoint: null handle" );
+    if ( !real_in || !spectrum_out || real_in == spectrum_out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_c2r_adjoint: bad data pointers" );
+    return run_passes( h->fwd, h->tb, h->prec, real_in, spectrum_out, h->work, h->stream, false, sync != 0,
+                       "ffts_c2r_adjoint", 1 );
+}
+
+static int spectral_launch_check( fftm_mi_ffts h, bool sync, const char *what )
+{
+    hipError_t e = hipGetLastError();
+    if ( e != hipSuccess )
+        return fail( FFTM_MI_EXEC_FAILED, std::string( what ) + ": launch failed: " + hipGetErrorString( e ) );
+    if ( sync )
+    {
+        e = hipDeviceSynchronize();
+        if ( e != hipSuccess )
+            return fail( FFTM_MI_EXEC_FAILED, std::string( what ) + ": synchronization failed: " + hipGetErrorString( e ) );
+    }
+    (void)h;
+    return ok();
+}
+
+int fftm_mi_ffts_spectral_multiply( fftm_mi_ffts h, const void *in, void *out, const void *filter,
+                                    const void *shell_coeffs, int32_t nshells, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_spectral_multiply: null handle" );
+    if ( !in || !out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_spectral_multiply: null data pointer" );
+    if ( ( filter == nullptr ) == ( shell_coeffs == nullptr ) )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_spectral_multiply: pass exactly one of filter / shell_coeffs" );
+    if ( shell_coeffs && ( nshells < 1 || nshells > MAX_SHELLS ) )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_spectral_multiply: nshells must be in 1..1024" );
+    spec_geom g = h->sg;
+    g.nshells   = shell_coeffs ? nshells : 1;
+    g.kscale    = h->sg.kscale * (double)g.nshells;
+    const int64_t blocks = std::min<int64_t>( ( g.total + 255 ) / 256, 8192 );
+    if ( h->prec == FFTM_MI_F32 )
+        hipLaunchKernelGGL( k_spectral_multiply<float>, dim3( (unsigned)blocks ), dim3( 256 ), 0, h->stream, g,
+                            static_cast<const cpx<float> *>( in ), static_cast<cpx<float> *>( out ),
+                            static_cast<const float *>( filter ), static_cast<const float *>( shell_coeffs ) );
+    else
+        hipLaunchKernelGGL( k_spectral_multiply<double>, dim3( (unsigned)blocks ), dim3( 256 ), 0, h->stream, g,
+                            static_cast<const cpx<double> *>( in ), static_cast<cpx<double> *>( out ),
+                            static_cast<const double *>( filter ), static_cast<const double *>( shell_coeffs ) );
+    return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_spectral_multiply" );
+}
+
+int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, const void *spectrum, void *grad,
+                                 int32_t nshells, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_shell_gradient: null handle" );
+    if ( !grad_spectrum || !spectrum || !grad )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_shell_gradient: null data pointer" );
+    if ( nshells < 1 || nshells > MAX_SHELLS )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_shell_gradient: nshells must be in 1..1024" );
+    if ( !h->partials )
+    {
+        const hipError_t e = hipMalloc( &h->partials, sizeof( double ) * SHELL_BLOCKS * MAX_SHELLS );
+        if ( e != hipSuccess )
+        {
+            h->partials = nullptr;
+            return fail( FFTM_MI_ALLOC_FAILED, std::string( "fftm_mi_ffts_shell_gradient: " ) + hipGetErrorString( e ) );
+        }
+    }
+    spec_geom g = h->sg;
+    g.nshells   = nshells;
+    g.kscale    = h->sg.kscale * (double)nshells;
+    const size_t lds = sizeof( double ) * (size_t)nshells;
+    if ( h->prec == FFTM_MI_F32 )
+    {
+        hipLaunchKernelGGL( k_shell_partials<float>, dim3( SHELL_BLOCKS ), dim3( 256 ), lds, h->stream, g,
+                            static_cast<const cpx<float> *>( grad_spectrum ), static_cast<const cpx<float> *>( spectrum ),
+                            h->partials );
+        hipLaunchKernelGGL( k_shell_reduce<float>, dim3( ( nshells + 255 ) / 256 ), dim3( 256 ), 0, h->stream,
+                            h->partials, SHELL_BLOCKS, nshells, static_cast<float *>( grad ) );
+    }
+    else
+    {
+        hipLaunchKernelGGL( k_shell_partials<double>, dim3( SHELL_BLOCKS ), dim3( 256 ), lds, h->stream, g,
+                            static_cast<const cpx<double> *>( grad_spectrum ),
+                            static_cast<const cpx<double> *>( spectrum ), h->partials );
+        hipLaunchKernelGGL( k_shell_reduce<double>, dim3( ( nshells + 255 ) / 256 ), dim3( 256 ), 0, h->stream,
+                            h->partials, SHELL_BLOCKS, nshells, static_cast<double *>( grad ) );
+    }
+    return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_shell_gradient" );
 }
 
 int fftm_mi_ffts_destroy( fftm_mi_ffts h )

@@ -52,8 +52,8 @@ struct pass_params
     int64_t       is0, is1, is2;
     int64_t       os0, os1, os2;
     int32_t       store_b; // 1: store with the column-fastest mapping, 0: line-fastest
-    int32_t       pad_;
-    T             scale; // multiplies every output (1 for the plain transforms)
+    int32_t       wmode;   // real-axis bin weights: 0 none, 1 R2C output x w_k, 2 C2R input / w_k
+    T             scale;   // multiplies every output (1 for the plain transforms)
 };
 
 // Runtime description for the generic (any-radix) kernel.
@@ -232,6 +232,12 @@ __device__ __forceinline__ cpx<T> load_line( const pass_params<T> &p, int64_t ia
             a.y = 0;
             b.y = 0;
         }
+        else if ( p.wmode == 2 )
+        {
+            // adjoint of R2C: interior bins of the half spectrum carry weight 1/2
+            a = scl( a, (T)0.5 );
+            b = scl( b, (T)0.5 );
+        }
         // z = A~ + i B~
         return mk<T>( a.x - b.y, a.y + b.x );
     }
@@ -346,9 +352,10 @@ struct pass_kernel
             T *dst = nullptr;
             (void)dst;
             cpx<T> *out = static_cast<cpx<T> *>( p.out );
-            const T h   = (T)0.5 * p.scale;
             for ( int k = t2; k <= N / 2; k += TPC )
             {
+                // adjoint of C2R: interior bins of the half spectrum carry weight 2
+                const T h  = ( p.wmode == 1 && k != 0 && 2 * k != N ) ? p.scale : (T)0.5 * p.scale;
                 const C zk = x.lds[L( k, c2 )];
                 const C zm = x.lds[L( k == 0 ? 0 : N - k, c2 )];
                 if ( va )
@@ -549,9 +556,9 @@ __global__ void __launch_bounds__( GEN_THREADS )
     if constexpr ( KIND == KIND_R2C )
     {
         cpx<T> *out = static_cast<cpx<T> *>( p.out );
-        const T h   = (T)0.5 * p.scale;
         for ( int k = t; k <= N / 2; k += tpc )
         {
+            const T      h  = ( p.wmode == 1 && k != 0 && 2 * k != N ) ? p.scale : (T)0.5 * p.scale;
             const cpx<T> zk = src[k * tw_cols + c];
             const cpx<T> zm = src[( k == 0 ? 0 : N - k ) * tw_cols + c];
             if ( va )

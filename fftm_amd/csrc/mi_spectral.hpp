@@ -1,0 +1,91 @@
+// Diagonal operators on an ffts half spectrum (no reference counterpart as a library
+// entry point; the reference applies such operators in caller functors, e.g.
+// solve_poisson_3d_functor, versioned_ffts_3d_test_driver.h:128-154, and the
+// spectral kernels of examples/turbulence/source/spacetime_kernels.h:66-145).
+//
+// Wavenumbers follow the reference drivers: k = i <= n/2 ? i : i - n on full axes
+// (Nyquist positive), k = i on the halved real axis.  A radial shell filter uses
+// S shells of equal width over [0, kmax], kmax = sqrt(sum_a (n_a/2)^2):
+//   shell(k) = min(S - 1, floor(S * |k| / kmax)).
+#pragma once
+
+#include "mi_complex.hpp"
+
+namespace fftm_mi
+{
+
+struct spec_geom
+{
+    int32_t nd;        // 3 or 4
+    int32_t half_axis; // logical index of the halved (real) axis
+    int64_t ext[4];    // memory extents, slowest..fastest
+    int32_t ax[4];     // logical axis of each memory axis
+    int64_t n[4];      // full logical lengths per logical axis
+    int64_t total;     // number of complex bins
+    double  kscale;    // S / kmax
+    int32_t nshells;
+};
+
+__device__ __forceinline__ int spec_shell( const spec_geom &g, int64_t i )
+{
+    double  k2 = 0.0;
+    int64_t r  = i;
+    for ( int m = g.nd - 1; m >= 0; --m )
+    {
+        const int64_t idx = r % g.ext[m];
+        r /= g.ext[m];
+        const int     a   = g.ax[m];
+        const int64_t na  = g.n[a];
+        const int64_t k   = ( a == g.half_axis || idx <= na / 2 ) ? idx : idx - na;
+        k2 += (double)k * (double)k;
+    }
+    int s = (int)( sqrt( k2 ) * g.kscale );
+    return s < g.nshells - 1 ? s : g.nshells - 1;
+}
+
+// out[i] = H_i * in[i];  H_i = filter[i] (per bin, real) or coeffs[shell(i)]
+template <typename T>
+__global__ void __launch_bounds__( 256 )
+    k_spectral_multiply( const spec_geom g, const cpx<T> *in, cpx<T> *out, const T *filter, const T *coeffs )
+{
+    for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < g.total; i += (int64_t)gridDim.x * 256 )
+    {
+        const T h = filter ? filter[i] : coeffs[spec_shell( g, i )];
+        out[i]    = scl( in[i], h );
+    }
+}
+
+// partial[b * S + s] = sum over this block's bins in shell s of Re(conj(G_i) X_i), in FP64
+template <typename T>
+__global__ void __launch_bounds__( 256 )
+    k_shell_partials( const spec_geom g, const cpx<T> *G, const cpx<T> *X, double *partial )
+{
+    extern __shared__ double acc[];
+    for ( int s = threadIdx.x; s < g.nshells; s += 256 )
+        acc[s] = 0.0;
+    __syncthreads();
+    for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < g.total; i += (int64_t)gridDim.x * 256 )
+    {
+        const cpx<T> a = G[i], b = X[i];
+        const double v = (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+        atomicAdd( &acc[spec_shell( g, i )], v );
+    }
+    __syncthreads();
+    for ( int s = threadIdx.x; s < g.nshells; s += 256 )
+        partial[(int64_t)blockIdx.x * g.nshells + s] = acc[s];
+}
+
+// grad[s] = sum_b partial[b * S + s], in a fixed order
+template <typename T>
+__global__ void __launch_bounds__( 256 ) k_shell_reduce( const double *partial, int nblocks, int nshells, T *grad )
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if ( s >= nshells )
+        return;
+    double sum = 0.0;
+    for ( int b = 0; b < nblocks; ++b )
+        sum += partial[(int64_t)b * nshells + s];
+    grad[s] = (T)sum;
+}
+
+} // namespace fftm_mi

@@ -412,6 +412,60 @@ class ffts:
             else:
                 self._backward_4d_staged(inp, out)
 
+    # -------------------------------------------- adjoints / spectral operators
+    # Spectrum inner product <X, Y> = Re sum conj(X) Y.  With w_k = 1 on the DC and
+    # Nyquist planes of the real axis and 2 elsewhere: R2C^T(G) = C2R(G / w) and
+    # C2R^T(v) = w R2C(v) (fused into the real-axis pass).  Fused execution only.
+    def _need_handle(self, what):
+        if self._handle is None:
+            raise LogicError(f"ffts::{what}: available with execution='fused' for 3D/4D")
+
+    def r2c_adjoint(self, spectrum, real_out):
+        """R2C^T: spectrum (complex layout) -> real layout."""
+        self._need_handle("r2c_adjoint")
+        self._check(spectrum, self.complex_shape(), self.complex, "r2c_adjoint input")
+        self._check(real_out, self.real_shape(), self.real, "r2c_adjoint output")
+        _lib.check(_lib.lib().fftm_mi_ffts_r2c_adjoint(self._handle, device_ptr(spectrum), device_ptr(real_out),
+                                                       1 if self._options.sync else 0), "fftm_mi_ffts_r2c_adjoint")
+
+    def c2r_adjoint(self, real_in, spectrum_out):
+        """C2R^T: real layout -> spectrum (complex layout)."""
+        self._need_handle("c2r_adjoint")
+        self._check(real_in, self.real_shape(), self.real, "c2r_adjoint input")
+        self._check(spectrum_out, self.complex_shape(), self.complex, "c2r_adjoint output")
+        _lib.check(_lib.lib().fftm_mi_ffts_c2r_adjoint(self._handle, device_ptr(real_in), device_ptr(spectrum_out),
+                                                       1 if self._options.sync else 0), "fftm_mi_ffts_c2r_adjoint")
+
+    def spectral_multiply(self, inp, out, filter=None, shell_coeffs=None):
+        """out = H * inp with a real per-bin filter or radial shell coefficients."""
+        self._need_handle("spectral_multiply")
+        self._check(inp, self.complex_shape(), self.complex, "spectral_multiply input")
+        self._check(out, self.complex_shape(), self.complex, "spectral_multiply output")
+        ns = 0
+        if filter is not None:
+            self._check(filter, self.complex_shape(), self.real, "spectral_multiply filter")
+        if shell_coeffs is not None:
+            if shell_coeffs.dtype != self.real or not shell_coeffs.is_cuda or shell_coeffs.dim() != 1:
+                raise ValueError("spectral_multiply: shell_coeffs must be a 1D GPU tensor of the real type")
+            ns = shell_coeffs.numel()
+        _lib.check(_lib.lib().fftm_mi_ffts_spectral_multiply(
+            self._handle, device_ptr(inp), device_ptr(out), device_ptr(filter) if filter is not None else None,
+            device_ptr(shell_coeffs) if shell_coeffs is not None else None, ns, 1 if self._options.sync else 0),
+            "fftm_mi_ffts_spectral_multiply")
+
+    def shell_gradient(self, grad_spectrum, spectrum, nshells):
+        """g[s] = sum_{k in shell s} Re(conj(G_k) X_k)."""
+        import torch
+        self._need_handle("shell_gradient")
+        self._check(grad_spectrum, self.complex_shape(), self.complex, "shell_gradient G")
+        self._check(spectrum, self.complex_shape(), self.complex, "shell_gradient X")
+        g = torch.empty(nshells, dtype=self.real, device="cuda")
+        _lib.check(_lib.lib().fftm_mi_ffts_shell_gradient(self._handle, device_ptr(grad_spectrum),
+                                                          device_ptr(spectrum), device_ptr(g), int(nshells),
+                                                          1 if self._options.sync else 0),
+                   "fftm_mi_ffts_shell_gradient")
+        return g
+
     # 4D stage sequences (ffts.hpp:809-867); transposes as dst = transpose(src, perm)
     # on memory arrays (the arranger algebra of direct_transpose_4d.h:42-120).
     def _tr(self, src, dst, perm):

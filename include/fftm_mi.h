@@ -169,6 +169,37 @@ int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, cons
 int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, int64_t *bytes );
 int fftm_mi_ffts_destroy( fftm_mi_ffts h );
 
+/*
+ * Adjoints and diagonal spectral operators (a capability the reference leaves to its
+ * callers' functors, e.g. solve_poisson_3d_functor in
+ * source/tests/detail/versioned_ffts_3d_test_driver.h:128-154; these entry points have no
+ * counterpart in the reference API).  The spectrum is treated as a real vector space with
+ * <X, Y> = Re sum conj(X_k) Y_k.  With w_k = 1 on the DC and (even n) Nyquist planes of
+ * the real axis and w_k = 2 elsewhere, C2R(X) = Re sum_k w_k X_k e^{+i k.x}, hence
+ *   R2C^T(G) = C2R(G / w)     (a Hermitian-weighted C2R)
+ *   C2R^T(v) = w * R2C(v).
+ * r2c_adjoint needs the work area like backward; both run the same passes as
+ * backward / forward with the weights folded into the real-axis pass.
+ */
+int fftm_mi_ffts_r2c_adjoint( fftm_mi_ffts h, const void *spectrum_in, void *real_out, int32_t sync );
+int fftm_mi_ffts_c2r_adjoint( fftm_mi_ffts h, const void *real_in, void *spectrum_out, int32_t sync );
+
+/*
+ * out_k = H_k in_k on the handle's spectrum layout (in == out allowed).  H is real:
+ * either per bin (filter: one T per complex bin, spectrum layout) or radial
+ * (shell_coeffs[nshells]: H_k = shell_coeffs[shell(k)], shell(k) = min(S-1,
+ * floor(S |k| / kmax)), kmax = sqrt(sum_a (n_a/2)^2), k = i <= n/2 ? i : i - n on full
+ * axes and k = i on the real axis).  Exactly one of filter / shell_coeffs is non-NULL.
+ * The operator is self-adjoint, so it also serves as its own adjoint.
+ */
+int fftm_mi_ffts_spectral_multiply( fftm_mi_ffts h, const void *in, void *out, const void *filter,
+                                    const void *shell_coeffs, int32_t nshells, int32_t sync );
+
+/* grad[s] = sum_{k in shell s} Re(conj(G_k) X_k) (gradient of a loss w.r.t. the shell
+ * coefficients, G = dL/dZ for Z = H X), accumulated in FP64, written as T[nshells]. */
+int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, const void *spectrum, void *grad,
+                                 int32_t nshells, int32_t sync );
+
 /* Last error message of the calling thread ("" when none). */
 const char *fftm_mi_last_error( void );
 

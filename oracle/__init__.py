@@ -21,6 +21,7 @@ from fastest to slowest, so its memory order is the reversed list.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -410,3 +411,41 @@ def rel_errors(actual: np.ndarray, expected: np.ndarray):
     rms = nd / ne if ne > 0 else nd
     mx = md / me if me > 0 else md
     return rms, mx
+
+
+# --------------------------------------------------------------------------
+# Spectral operators (restated for the adjoint / filter tests)
+# --------------------------------------------------------------------------
+
+def wavenumbers(n: int, halved: bool) -> np.ndarray:
+    """k = i <= n/2 ? i : i - n on full axes (Nyquist positive), k = i on the halved axis
+    (versioned_ffts_3d_test_driver.h:140-142)."""
+    if halved:
+        return np.arange(n // 2 + 1)
+    i = np.arange(n)
+    return np.where(i <= n // 2, i, i - n)
+
+
+def shell_index_logical(dims, nshells: int) -> np.ndarray:
+    """shell(k) = min(S-1, floor(S |k| / kmax)), kmax = sqrt(sum (n_a/2)^2), on the logical
+    half spectrum (last axis halved); returned as an int array of the logical half shape."""
+    d = len(dims)
+    ks = [wavenumbers(n, a == d - 1) for a, n in enumerate(dims)]
+    grids = np.meshgrid(*ks, indexing="ij")
+    k2 = np.zeros(grids[0].shape, dtype=np.float64)
+    for g in grids:
+        k2 += g.astype(np.float64) ** 2
+    kmax2 = float(sum((n // 2) ** 2 for n in dims))
+    kscale = (1.0 / math.sqrt(kmax2)) * float(nshells)
+    s = np.floor(np.sqrt(k2) * kscale).astype(np.int64)
+    return np.minimum(s, nshells - 1)
+
+
+def realaxis_weights_logical(dims) -> np.ndarray:
+    """w_k = 1 on the DC and (even n) Nyquist planes of the real axis, 2 elsewhere."""
+    n = dims[-1]
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = 1.0
+    if n % 2 == 0:
+        w[-1] = 1.0
+    return np.broadcast_to(w, tuple(dims[:-1]) + (n // 2 + 1,))
