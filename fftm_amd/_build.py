@@ -16,7 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfftm_mi.so")
 
-SOURCES = ["mi_kernels_f32.hip", "mi_kernels_f64.hip", "mi_host.hip"]
+SOURCES = ["mi_kernels_f32_a.hip", "mi_kernels_f32_b.hip", "mi_kernels_f32_c.hip", "mi_kernels_f32_d.hip",
+           "mi_kernels_f64_a.hip", "mi_kernels_f64_b.hip", "mi_kernels_f64_c.hip", "mi_kernels_f64_d.hip",
+           "mi_kernels_dispatch.hip", "mi_host.hip"]
 
 # Packed FP32 VALU ops are disabled for every HIP translation unit (device side).
 HIPFLAGS = [

@@ -92,6 +92,8 @@ SIGNATURES = {
     "fftm_mi_ffts_num_passes": (ctypes.c_int, [_vp, _i32p, _i32p]),
     "fftm_mi_ffts_run_pass": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _i32]),
     "fftm_mi_ffts_pass_bytes": (ctypes.c_int, [_vp, _i32, _i32, _i64p]),
+    "fftm_mi_ffts_set_pass_variant": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
+    "fftm_mi_ffts_get_pass_variant": (ctypes.c_int, [_vp, _i32, _i32, _i32p]),
     "fftm_mi_ffts_destroy": (ctypes.c_int, [_vp]),
     "fftm_mi_ffts_r2c_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
     "fftm_mi_ffts_c2r_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _i32]),

@@ -64,6 +64,7 @@ struct pass_rt
     radix_list rl{};
     int        gen_cols = 1;
     int        wmode    = 0; // real-axis bin weights (adjoints), see pass_params::wmode
+    int        var      = 0; // tile-shape variant of the fixed-length kernel
     double     scale    = 1.0;
 };
 
@@ -253,10 +254,10 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.scale   = (T)r.scale;
     if constexpr ( std::is_same<T, float>::value )
         return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
-                         : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, s );
+                         : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
     else
         return r.generic ? launch_generic_f64( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
-                         : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, s );
+                         : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
 }
 
 int run_passes( const std::vector<pass_rt> &passes, const tables &tb, int prec, const void *in, void *out,
@@ -1173,6 +1174,30 @@ int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, cons
                             h->partials, SHELL_BLOCKS, nshells, static_cast<double *>( grad ) );
     }
     return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_shell_gradient" );
+}
+
+int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_set_pass_variant: null handle" );
+    std::vector<pass_rt> &v = backward ? h->bwd : h->fwd;
+    if ( index < 0 || index >= (int32_t)v.size() )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: pass index out of range" );
+    if ( variant < 0 || variant >= NUM_VARIANTS )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: variant out of range" );
+    v[(size_t)index].var = variant;
+    return ok();
+}
+
+int fftm_mi_ffts_get_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t *variant )
+{
+    if ( !h || !variant )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_get_pass_variant: null argument" );
+    const std::vector<pass_rt> &v = backward ? h->bwd : h->fwd;
+    if ( index < 0 || index >= (int32_t)v.size() )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_get_pass_variant: pass index out of range" );
+    *variant = v[(size_t)index].var;
+    return ok();
 }
 
 int fftm_mi_ffts_destroy( fftm_mi_ffts h )

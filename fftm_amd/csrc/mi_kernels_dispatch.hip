@@ -1,0 +1,73 @@
+// Dispatch of the fixed-length pass kernels (one definition per length lives in the
+// mi_kernels_<prec>_<group>.hip units) and the generic kernel launchers.
+#include "mi_kernels_impl.hpp"
+
+#define FFTM_MI_LENGTHS( X, T, TN )                                                                                    \
+    X( T, TN, 2 ) X( T, TN, 4 ) X( T, TN, 8 ) X( T, TN, 16 ) X( T, TN, 32 ) X( T, TN, 64 ) X( T, TN, 128 )             \
+        X( T, TN, 256 ) X( T, TN, 512 ) X( T, TN, 1024 ) X( T, TN, 2048 ) X( T, TN, 4096 )
+
+#define FFTM_MI_DECLARE( T, TN, N )                                                                                    \
+    hipError_t launch_##TN##_n##N( const pass_params<T> &, int, bool, bool, int, hipStream_t );                       \
+    int        tile_cols_##TN##_n##N( bool, int );
+
+#define FFTM_MI_CASE_LAUNCH( T, TN, N )                                                                                \
+    case N: return launch_##TN##_n##N( p, kind, inv, loadb, var, s );
+
+#define FFTM_MI_CASE_COLS( T, TN, N )                                                                                  \
+    case N: return tile_cols_##TN##_n##N( loadb, var );
+
+namespace fftm_mi
+{
+
+FFTM_MI_LENGTHS( FFTM_MI_DECLARE, float, f32 )
+FFTM_MI_LENGTHS( FFTM_MI_DECLARE, double, f64 )
+
+hipError_t launch_pass_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s )
+{
+    switch ( N )
+    {
+        FFTM_MI_LENGTHS( FFTM_MI_CASE_LAUNCH, float, f32 )
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s )
+{
+    switch ( N )
+    {
+        FFTM_MI_LENGTHS( FFTM_MI_CASE_LAUNCH, double, f64 )
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int fixed_tile_cols_f32( int N, bool loadb, int var )
+{
+    switch ( N )
+    {
+        FFTM_MI_LENGTHS( FFTM_MI_CASE_COLS, float, f32 )
+    default: return 0;
+    }
+}
+
+int fixed_tile_cols_f64( int N, bool loadb, int var )
+{
+    switch ( N )
+    {
+        FFTM_MI_LENGTHS( FFTM_MI_CASE_COLS, double, f64 )
+    default: return 0;
+    }
+}
+
+hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
+                               hipStream_t s )
+{
+    return detail::launch_generic_t<float>( p, rl, N, cols, kind, inv, s );
+}
+
+hipError_t launch_generic_f64( const pass_params<double> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
+                               hipStream_t s )
+{
+    return detail::launch_generic_t<double>( p, rl, N, cols, kind, inv, s );
+}
+
+} // namespace fftm_mi

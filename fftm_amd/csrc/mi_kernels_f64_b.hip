@@ -1,0 +1,4 @@
+// F64 pass kernels, lengths 512.
+#include "mi_kernels_impl.hpp"
+
+FFTM_MI_DEFINE_LENGTH( double, f64, 512 )

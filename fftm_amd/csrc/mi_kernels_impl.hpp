@@ -1,5 +1,5 @@
-// Instantiation and dispatch of the pass kernels for one precision T.
-// Included once per precision by mi_kernels_f32.hip / mi_kernels_f64.hip.
+// Instantiation and dispatch of the pass kernels for one precision T and one length N.
+// Included by the per-length translation units (mi_kernels_<prec>_<group>.hip).
 #pragma once
 
 #include "mi_launch.hpp"
@@ -9,83 +9,81 @@ namespace fftm_mi
 namespace detail
 {
 
-template <typename T, int N, int KIND, bool INV, bool LOADB>
+// tile-shape variants compiled for a length (small lengths have one)
+template <int N>
+constexpr int variants_for()
+{
+    return N >= 64 ? NUM_VARIANTS : 1;
+}
+
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR>
 hipError_t launch_fixed( const pass_params<T> &p, hipStream_t s )
 {
-    using K                = pass_kernel<T, N, KIND, INV, LOADB>;
-    const int64_t per      = (int64_t)K::TW * K::CPT;
-    const int64_t ntiles   = ( p.ncols + per - 1 ) / per;
+    using K              = pass_kernel<T, N, KIND, INV, LOADB, VAR>;
+    const int64_t per    = (int64_t)K::TW * K::CPT;
+    const int64_t ntiles = ( p.ncols + per - 1 ) / per;
     if ( ntiles <= 0 )
         return hipSuccess;
     if ( ntiles > 0x7fffffffLL )
         return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL( ( k_pass<T, N, KIND, INV, LOADB> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p );
+    hipLaunchKernelGGL( ( k_pass<T, N, KIND, INV, LOADB, VAR> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p );
     return hipGetLastError();
 }
 
-template <typename T, int N>
-hipError_t launch_n( const pass_params<T> &p, int kind, bool inv, bool loadb, hipStream_t s )
+template <typename T, int N, int VAR>
+hipError_t launch_nv( const pass_params<T> &p, int kind, bool inv, bool loadb, hipStream_t s )
 {
     if ( kind == KIND_C2C )
     {
         if ( inv )
-            return loadb ? launch_fixed<T, N, KIND_C2C, true, true>( p, s )
-                         : launch_fixed<T, N, KIND_C2C, true, false>( p, s );
-        return loadb ? launch_fixed<T, N, KIND_C2C, false, true>( p, s )
-                     : launch_fixed<T, N, KIND_C2C, false, false>( p, s );
+            return loadb ? launch_fixed<T, N, KIND_C2C, true, true, VAR>( p, s )
+                         : launch_fixed<T, N, KIND_C2C, true, false, VAR>( p, s );
+        return loadb ? launch_fixed<T, N, KIND_C2C, false, true, VAR>( p, s )
+                     : launch_fixed<T, N, KIND_C2C, false, false, VAR>( p, s );
     }
     if ( kind == KIND_R2C )
-        return loadb ? launch_fixed<T, N, KIND_R2C, false, true>( p, s )
-                     : launch_fixed<T, N, KIND_R2C, false, false>( p, s );
-    return loadb ? launch_fixed<T, N, KIND_C2R, true, true>( p, s ) : launch_fixed<T, N, KIND_C2R, true, false>( p, s );
-}
-
-template <typename T>
-hipError_t launch_pass_t( const pass_params<T> &p, int N, int kind, bool inv, bool loadb, hipStream_t s )
-{
-    switch ( N )
-    {
-    case 2: return launch_n<T, 2>( p, kind, inv, loadb, s );
-    case 4: return launch_n<T, 4>( p, kind, inv, loadb, s );
-    case 8: return launch_n<T, 8>( p, kind, inv, loadb, s );
-    case 16: return launch_n<T, 16>( p, kind, inv, loadb, s );
-    case 32: return launch_n<T, 32>( p, kind, inv, loadb, s );
-    case 64: return launch_n<T, 64>( p, kind, inv, loadb, s );
-    case 128: return launch_n<T, 128>( p, kind, inv, loadb, s );
-    case 256: return launch_n<T, 256>( p, kind, inv, loadb, s );
-    case 512: return launch_n<T, 512>( p, kind, inv, loadb, s );
-    case 1024: return launch_n<T, 1024>( p, kind, inv, loadb, s );
-    case 2048: return launch_n<T, 2048>( p, kind, inv, loadb, s );
-    case 4096: return launch_n<T, 4096>( p, kind, inv, loadb, s );
-    default: return hipErrorInvalidValue;
-    }
+        return loadb ? launch_fixed<T, N, KIND_R2C, false, true, VAR>( p, s )
+                     : launch_fixed<T, N, KIND_R2C, false, false, VAR>( p, s );
+    return loadb ? launch_fixed<T, N, KIND_C2R, true, true, VAR>( p, s )
+                 : launch_fixed<T, N, KIND_C2R, true, false, VAR>( p, s );
 }
 
 template <typename T, int N>
-int tile_cols_n( bool loadb )
+hipError_t launch_n( const pass_params<T> &p, int kind, bool inv, bool loadb, int var, hipStream_t s )
 {
-    return loadb ? pass_kernel<T, N, KIND_C2C, false, true>::TW : pass_kernel<T, N, KIND_C2C, false, false>::TW;
+    if constexpr ( variants_for<N>() > 1 )
+    {
+        switch ( var )
+        {
+        case 1: return launch_nv<T, N, 1>( p, kind, inv, loadb, s );
+        case 2: return launch_nv<T, N, 2>( p, kind, inv, loadb, s );
+        case 3: return launch_nv<T, N, 3>( p, kind, inv, loadb, s );
+        default: break;
+        }
+    }
+    return launch_nv<T, N, 0>( p, kind, inv, loadb, s );
 }
 
-template <typename T>
-int tile_cols_t( int N, bool loadb )
+template <typename T, int N, int VAR>
+int tile_cols_nv( bool loadb )
 {
-    switch ( N )
+    return loadb ? pass_kernel<T, N, KIND_C2C, false, true, VAR>::TW : pass_kernel<T, N, KIND_C2C, false, false, VAR>::TW;
+}
+
+template <typename T, int N>
+int tile_cols_n( bool loadb, int var )
+{
+    if constexpr ( variants_for<N>() > 1 )
     {
-    case 2: return tile_cols_n<T, 2>( loadb );
-    case 4: return tile_cols_n<T, 4>( loadb );
-    case 8: return tile_cols_n<T, 8>( loadb );
-    case 16: return tile_cols_n<T, 16>( loadb );
-    case 32: return tile_cols_n<T, 32>( loadb );
-    case 64: return tile_cols_n<T, 64>( loadb );
-    case 128: return tile_cols_n<T, 128>( loadb );
-    case 256: return tile_cols_n<T, 256>( loadb );
-    case 512: return tile_cols_n<T, 512>( loadb );
-    case 1024: return tile_cols_n<T, 1024>( loadb );
-    case 2048: return tile_cols_n<T, 2048>( loadb );
-    case 4096: return tile_cols_n<T, 4096>( loadb );
-    default: return 0;
+        switch ( var )
+        {
+        case 1: return tile_cols_nv<T, N, 1>( loadb );
+        case 2: return tile_cols_nv<T, N, 2>( loadb );
+        case 3: return tile_cols_nv<T, N, 3>( loadb );
+        default: break;
+        }
     }
+    return tile_cols_nv<T, N, 0>( loadb );
 }
 
 template <typename T>
@@ -117,3 +115,17 @@ hipError_t launch_generic_t( const pass_params<T> &p, const radix_list &rl, int 
 
 } // namespace detail
 } // namespace fftm_mi
+
+// One length N for precision T: defines launch_<tn>_n<N> and tile_cols_<tn>_n<N>.
+#define FFTM_MI_DEFINE_LENGTH( T, TN, N )                                                                              \
+    namespace fftm_mi                                                                                                  \
+    {                                                                                                                  \
+    hipError_t launch_##TN##_n##N( const pass_params<T> &p, int kind, bool inv, bool loadb, int var, hipStream_t s ) \
+    {                                                                                                                  \
+        return detail::launch_n<T, N>( p, kind, inv, loadb, var, s );                                                  \
+    }                                                                                                                  \
+    int tile_cols_##TN##_n##N( bool loadb, int var )                                                                   \
+    {                                                                                                                  \
+        return detail::tile_cols_n<T, N>( loadb, var );                                                                \
+    }                                                                                                                  \
+    }

@@ -17,11 +17,11 @@ inline bool fixed_supported( int64_t N )
 }
 
 // columns per workgroup of a fixed-N kernel (complex lines)
-int fixed_tile_cols_f32( int N, bool loadb );
-int fixed_tile_cols_f64( int N, bool loadb );
+int fixed_tile_cols_f32( int N, bool loadb, int var );
+int fixed_tile_cols_f64( int N, bool loadb, int var );
 
-hipError_t launch_pass_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, hipStream_t s );
-hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool inv, bool loadb, hipStream_t s );
+hipError_t launch_pass_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s );
+hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s );
 hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
                                hipStream_t s );
 hipError_t launch_generic_f64( const pass_params<double> &p, const radix_list &rl, int N, int cols, int kind, bool inv,

@@ -140,7 +140,13 @@ constexpr int lds_limit_bytes() { return 65536; }
 
 constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 
-constexpr geom pick_geom( int N, int csize, bool loadb )
+// var selects a tile-shape variant (0 = default); the tuner measures them per pass:
+//   1: half the columns per workgroup (smaller LDS tile, more workgroups per CU)
+//   2: twice the threads per column (fewer elements per thread, larger workgroup)
+//   3: both (same workgroup size, half the tile)
+constexpr int NUM_VARIANTS = 4;
+
+constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 {
     const rplan p  = make_rplan( N );
     const int   rm = rplan_max( p );
@@ -180,6 +186,10 @@ constexpr geom pick_geom( int N, int csize, bool loadb )
         g.tpc = tpc;
         g.pad = pad;
     }
+    if ( ( var == 1 || var == 3 ) && g.tw > 1 )
+        g.tw /= 2;
+    if ( ( var == 2 || var == 3 ) && g.tpc * 2 <= hi && g.tpc * 2 * g.tw <= 1024 )
+        g.tpc *= 2;
     return g;
 }
 
@@ -266,12 +276,12 @@ __device__ __forceinline__ void store_line( const pass_params<T> &p, int64_t oa,
 // ---------------------------------------------------------------------------
 // the compile-time-N pass kernel
 // ---------------------------------------------------------------------------
-template <typename T, int N, int KIND, bool INV, bool LOADB>
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR = 0>
 struct pass_kernel
 {
     using C                    = cpx<T>;
     static constexpr rplan RP  = make_rplan( N );
-    static constexpr geom  G   = pick_geom( N, (int)sizeof( C ), LOADB );
+    static constexpr geom  G   = pick_geom( N, (int)sizeof( C ), LOADB, VAR );
     static constexpr int   TW  = G.tw;
     static constexpr int   TPC = G.tpc;
     static constexpr int   NT  = TW * TPC;
@@ -462,11 +472,11 @@ struct pass_kernel
     }
 };
 
-template <typename T, int N, int KIND, bool INV, bool LOADB>
-__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB>::NT ) )
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR>
+__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB, VAR>::NT ) )
     k_pass( const pass_params<T> p )
 {
-    pass_kernel<T, N, KIND, INV, LOADB>::run( p );
+    pass_kernel<T, N, KIND, INV, LOADB, VAR>::run( p );
 }
 
 // ---------------------------------------------------------------------------
