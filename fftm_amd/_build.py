@@ -18,12 +18,13 @@ LIB = os.path.join(HERE, "libfftm_mi.so")
 
 SOURCES = ["mi_kernels_f32_a.hip", "mi_kernels_f32_b.hip", "mi_kernels_f32_c.hip", "mi_kernels_f32_d.hip",
            "mi_kernels_f64_a.hip", "mi_kernels_f64_b.hip", "mi_kernels_f64_c.hip", "mi_kernels_f64_d.hip",
-           "mi_kernels_dispatch.hip", "mi_host.hip"]
+           "mi_kernels_dispatch.hip", "mi_kernels_plane.hip", "mi_host.hip"]
 
 # Packed FP32 VALU ops are disabled for every HIP translation unit (device side).
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
 ]
 

@@ -39,17 +39,20 @@ __device__ __forceinline__ cpx<T> sub( cpx<T> a, cpx<T> b )
     return mk<T>( a.x - b.x, a.y - b.y );
 }
 
+// Complex products use explicit fused multiply-adds (the library is compiled with
+// -ffp-contract=off), so every kernel instantiation rounds identically whatever the
+// unrolling: tile-shape variants stay bitwise equal.
 template <typename T>
 __device__ __forceinline__ cpx<T> mul( cpx<T> a, cpx<T> b )
 {
-    return mk<T>( a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x );
+    return mk<T>( __builtin_fma( a.x, b.x, -( a.y * b.y ) ), __builtin_fma( a.x, b.y, a.y * b.x ) );
 }
 
 // a * conj(b)
 template <typename T>
 __device__ __forceinline__ cpx<T> mulc( cpx<T> a, cpx<T> b )
 {
-    return mk<T>( a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y );
+    return mk<T>( __builtin_fma( a.x, b.x, a.y * b.y ), __builtin_fma( a.y, b.x, -( a.x * b.y ) ) );
 }
 
 template <typename T>

@@ -5,8 +5,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <map>
+#include <sstream>
 #include <memory>
 #include <new>
 #include <string>
@@ -66,6 +71,11 @@ struct pass_rt
     int        wmode    = 0; // real-axis bin weights (adjoints), see pass_params::wmode
     int        var      = 0; // tile-shape variant of the fixed-length kernel
     double     scale    = 1.0;
+    // plane pass (the real axis A of length N and axis B of length NB together)
+    bool       plane    = false;
+    int64_t    NB       = 0;
+    int64_t    ra = 0, rb = 0, ca = 0, cb = 0;      // real / complex strides along A and B
+    int64_t    nplanes = 0, pd0 = 1, ri0 = 0, ri1 = 0, ci0 = 0, ci1 = 0;
 };
 
 bool is_pow2( int64_t n )
@@ -232,8 +242,38 @@ struct tables
 // pass execution
 // ---------------------------------------------------------------------------
 template <typename T>
+hipError_t launch_plane_one( const pass_rt &r, const tables &tb, const void *src, void *dst, hipStream_t s )
+{
+    plane_params<T> p;
+    p.in      = src;
+    p.out     = dst;
+    p.twa     = static_cast<const cpx<T> *>( tb.by_n.at( r.N ) );
+    p.twb     = static_cast<const cpx<T> *>( tb.by_n.at( r.NB ) );
+    p.ra      = r.ra;
+    p.rb      = r.rb;
+    p.ca      = r.ca;
+    p.cb      = r.cb;
+    p.nplanes = r.nplanes;
+    p.d0      = r.pd0;
+    p.ri0     = r.ri0;
+    p.ri1     = r.ri1;
+    p.ci0     = r.ci0;
+    p.ci1     = r.ci1;
+    p.wmode   = r.wmode;
+    p.pad_    = 0;
+    p.scale   = (T)r.scale;
+    const bool fwd = r.kind == KIND_R2C;
+    if constexpr ( std::is_same<T, float>::value )
+        return launch_plane_f32( p, (int)r.N, (int)r.NB, fwd, s );
+    else
+        return launch_plane_f64( p, (int)r.N, (int)r.NB, fwd, s );
+}
+
+template <typename T>
 hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void *dst, hipStream_t s )
 {
+    if ( r.plane )
+        return launch_plane_one<T>( r, tb, src, dst, s );
     pass_params<T> p;
     p.in      = src;
     p.out     = dst;
@@ -292,12 +332,17 @@ int prepare_passes( std::vector<pass_rt> &passes, int prec, tables *tb )
 {
     for ( pass_rt &p : passes )
     {
-        const int rc = configure_kernel( p, prec );
-        if ( rc != FFTM_MI_SUCCESS )
-            return rc;
+        if ( !p.plane )
+        {
+            const int rc = configure_kernel( p, prec );
+            if ( rc != FFTM_MI_SUCCESS )
+                return rc;
+        }
         if ( tb )
         {
-            const int rt = tb->ensure( p.N );
+            int rt = tb->ensure( p.N );
+            if ( rt == FFTM_MI_SUCCESS && p.plane )
+                rt = tb->ensure( p.NB );
             if ( rt != FFTM_MI_SUCCESS )
                 return rt;
         }
@@ -607,6 +652,80 @@ int make_ffts_layout( int32_t layout, const int64_t *nn, ffts_layout &L )
         L.sc[3] = nx;
     }
     return ok();
+}
+
+// The axis paired with the real axis in a plane pass: the one that makes the plane
+// contiguous on at least one side (3D: y; 4D slab: z; 4D pencil / xzwy: x).
+int plane_partner( int32_t layout )
+{
+    switch ( layout )
+    {
+    case FFTM_MI_LAYOUT_3D: return 1;
+    case FFTM_MI_LAYOUT_4D_SLAB: return 2;
+    default: return 0;
+    }
+}
+
+// Replace the real-axis pass and the C2C pass along `B` by one plane pass when the
+// (B x A) half-spectrum plane fits in LDS and there are enough planes to fill the chip.
+void fuse_plane( const ffts_layout &L, int32_t layout, int prec, std::vector<pass_rt> &fwd, std::vector<pass_rt> &bwd )
+{
+    const int     d  = L.dim;
+    const int     A  = d - 1;
+    const int     B  = plane_partner( layout );
+    const int64_t na = L.n[A], nb = L.n[B];
+    if ( !plane_supported( na, nb, prec ) )
+        return;
+    int64_t nplanes = 1;
+    int     other[2], no = 0;
+    for ( int k = 0; k < d; ++k )
+        if ( k != A && k != B )
+        {
+            nplanes *= L.n[k];
+            other[no++] = k;
+        }
+    if ( nplanes < 512 )
+        return;
+    pass_rt p;
+    p.plane   = true;
+    p.N       = na;
+    p.NB      = nb;
+    p.ra      = L.sr[A];
+    p.rb      = L.sr[B];
+    p.ca      = L.sc[A];
+    p.cb      = L.sc[B];
+    p.nplanes = nplanes;
+    p.pd0     = no > 0 ? L.n[other[0]] : 1;
+    p.ri0     = no > 0 ? L.sr[other[0]] : 0;
+    p.ci0     = no > 0 ? L.sc[other[0]] : 0;
+    p.ri1     = no > 1 ? L.sr[other[1]] : 0;
+    p.ci1     = no > 1 ? L.sc[other[1]] : 0;
+    // forward: drop the real pass (first) and the C2C pass along B, plane first
+    std::vector<pass_rt> f;
+    pass_rt              pf = p;
+    pf.kind                 = KIND_R2C;
+    pf.src                  = BUF_IN;
+    pf.dst                  = BUF_OUT;
+    f.push_back( pf );
+    for ( size_t i = 1; i < fwd.size(); ++i )
+        if ( !( fwd[i].kind == KIND_C2C && fwd[i].N == nb && fwd[i].ies == L.sc[B] ) )
+            f.push_back( fwd[i] );
+    // backward: drop the C2C pass along B and the final C2R, plane last
+    std::vector<pass_rt> b;
+    for ( size_t i = 0; i + 1 < bwd.size(); ++i )
+        if ( !( bwd[i].kind == KIND_C2C && bwd[i].N == nb && bwd[i].ies == L.sc[B] ) )
+            b.push_back( bwd[i] );
+    pass_rt pb = p;
+    pb.kind    = KIND_C2R;
+    pb.inv     = true;
+    pb.dst     = BUF_OUT;
+    pb.src     = b.empty() ? BUF_IN : b.back().dst;
+    // the first remaining pass reads the user input
+    if ( !b.empty() )
+        b.front().src = BUF_IN;
+    b.push_back( pb );
+    fwd.swap( f );
+    bwd.swap( b );
 }
 
 int build_ffts_passes( const ffts_layout &L, bool overwrite, std::vector<pass_rt> &fwd, std::vector<pass_rt> &bwd,
@@ -937,6 +1056,8 @@ int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, 
     rc = build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, f, b, we );
     if ( rc != FFTM_MI_SUCCESS )
         return rc;
+    if ( !( flags & FFTM_MI_FLAG_NO_PLANE_FUSION ) )
+        fuse_plane( L, layout, precision, f, b );
     rc = prepare_passes( f, precision, nullptr );
     if ( rc == FFTM_MI_SUCCESS )
         rc = prepare_passes( b, precision, nullptr );
@@ -970,6 +1091,8 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
     make_ffts_layout( layout, n, L );
     int64_t we = 0;
     build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, p->fwd, p->bwd, we );
+    if ( !( flags & FFTM_MI_FLAG_NO_PLANE_FUSION ) )
+        fuse_plane( L, layout, precision, p->fwd, p->bwd );
     rc = prepare_passes( p->fwd, precision, &p->tb );
     if ( rc == FFTM_MI_SUCCESS )
         rc = prepare_passes( p->bwd, precision, &p->tb );
@@ -978,6 +1101,13 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
     p->work_size = ws;
     p->sg        = make_spec_geom( L );
     *h           = p.release();
+    if ( const char *cache = std::getenv( "FFTM_MI_AUTOTUNE_CACHE" ) )
+        if ( *cache )
+        {
+            int32_t applied = 0;
+            (void)fftm_mi_ffts_load_tuning( *h, cache, &applied );
+            g_err.clear();
+        }
     return ok();
 }
 
@@ -1065,7 +1195,9 @@ int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, in
     const pass_rt &p  = v[(size_t)index];
     const int64_t  cs = csize_of( h->prec ), rs = cs / 2;
     const int64_t  nh = p.N / 2 + 1;
-    if ( p.kind == KIND_C2C )
+    if ( p.plane )
+        *bytes = p.nplanes * p.NB * ( p.N * rs + nh * cs );
+    else if ( p.kind == KIND_C2C )
         *bytes = 2 * p.ncols * p.N * cs;
     else
         *bytes = p.ncols * ( p.N * rs + nh * cs );
@@ -1205,6 +1337,358 @@ int fftm_mi_ffts_destroy( fftm_mi_ffts h )
     if ( !h )
         return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_destroy: null handle" );
     delete h;
+    return ok();
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tile-shape tuning with the reference's autotune cache protocol
+// (source/fftm_autotune.hpp:126-169 KEY=VALUE cache, header line, tmp + rename;
+//  detail/fftm_autotune_hardware.inc:87-98, 285-337 FNV-1a-64 "v2:" signature;
+//  fftm_autotune_measure.hpp:293-400 median of iterations after warmup, per-candidate
+//  records, fastest valid candidate wins).  Here the candidates are the tile-shape
+//  variants of each pass; validity = output bitwise equal to variant 0.
+// ---------------------------------------------------------------------------
+namespace
+{
+
+std::string fnv1a64_hex( const std::string &v )
+{
+    uint64_t h = 14695981039346656037ull;
+    for ( unsigned char c : v )
+    {
+        h ^= (uint64_t)c;
+        h *= 1099511628211ull;
+    }
+    char buf[17];
+    std::snprintf( buf, sizeof( buf ), "%016llx", (unsigned long long)h );
+    return buf;
+}
+
+std::string trim_copy( const std::string &s )
+{
+    size_t a = s.find_first_not_of( " \t\r\n" );
+    if ( a == std::string::npos )
+        return "";
+    size_t b = s.find_last_not_of( " \t\r\n" );
+    return s.substr( a, b - a + 1 );
+}
+
+using cache_map = std::map<std::string, std::string>;
+
+// returns 0 ok, 1 missing file, -line for a malformed line
+int cache_load( const std::string &path, cache_map &out )
+{
+    std::ifstream in( path.c_str() );
+    if ( !in )
+        return 1;
+    std::string line;
+    int         no = 0;
+    while ( std::getline( in, line ) )
+    {
+        ++no;
+        line = trim_copy( line );
+        if ( line.empty() || line[0] == '#' )
+            continue;
+        const size_t eq = line.find( '=' );
+        if ( eq == std::string::npos )
+            return -no;
+        out[trim_copy( line.substr( 0, eq ) )] = trim_copy( line.substr( eq + 1 ) );
+    }
+    return 0;
+}
+
+bool cache_save( const std::string &path, const cache_map &m )
+{
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream o( tmp.c_str() );
+        if ( !o )
+            return false;
+        o << "# FFTM C++ autotune cache\n";
+        for ( const auto &kv : m )
+            o << kv.first << "=" << kv.second << "\n";
+    }
+    if ( std::rename( tmp.c_str(), path.c_str() ) != 0 )
+    {
+        std::remove( tmp.c_str() );
+        return false;
+    }
+    return true;
+}
+
+// canonical hardware string and signature of the current device
+bool hardware_signature( std::string &canonical, std::string &signature, std::string &arch, std::string &model )
+{
+    int dev = 0;
+    if ( hipGetDevice( &dev ) != hipSuccess )
+        return false;
+    hipDeviceProp_t pr;
+    if ( hipGetDeviceProperties( &pr, dev ) != hipSuccess )
+        return false;
+    int rt = 0, drv = 0;
+    (void)hipRuntimeGetVersion( &rt );
+    (void)hipDriverGetVersion( &drv );
+    arch  = pr.gcnArchName;
+    model = pr.name;
+    for ( char &c : model )
+        if ( c == ' ' || c == ';' || c == '=' )
+            c = '_';
+    std::ostringstream o;
+    o << "version=2;device_classes=" << arch << ";device_models=" << model << ";compute_units=" << pr.multiProcessorCount
+      << ";device_total_bytes=" << (unsigned long long)pr.totalGlobalMem << ";runtime_versions=" << rt
+      << ";driver_versions=" << drv;
+    canonical = o.str();
+    signature = "v2:" + fnv1a64_hex( canonical );
+    return true;
+}
+
+std::string problem_key( const fftm_mi_ffts_s &h )
+{
+    static const char *names[] = { "3D", "4D_PENCIL", "4D_SLAB", "4D_XZWY" };
+    std::ostringstream o;
+    o << "FFTM_MI_TILE_" << names[h.layout] << "_" << h.n[0] << "x" << h.n[1] << "x" << h.n[2];
+    if ( h.layout != FFTM_MI_LAYOUT_3D )
+        o << "x" << h.n[3];
+    o << ( h.prec == FFTM_MI_F32 ? "_F32" : "_F64" );
+    return o.str();
+}
+
+std::string pass_key( const std::string &prob, int bw, size_t i )
+{
+    return prob + ( bw ? "_BWD" : "_FWD" ) + std::to_string( i );
+}
+
+__global__ void k_count_diff( const uint32_t *a, const uint32_t *b, int64_t n, unsigned long long *count )
+{
+    unsigned long long c = 0;
+    for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256 )
+        c += a[i] != b[i] ? 1ull : 0ull;
+    if ( c )
+        atomicAdd( count, c );
+}
+
+} // namespace
+
+extern "C" {
+
+int fftm_mi_fnv1a64( const char *text, char *hex17 )
+{
+    if ( !text || !hex17 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_fnv1a64: null argument" );
+    const std::string h = fnv1a64_hex( text );
+    std::memcpy( hex17, h.c_str(), 17 );
+    return ok();
+}
+
+int fftm_mi_hardware_signature( char *buf, size_t len )
+{
+    std::string canon, sig, arch, model;
+    if ( !hardware_signature( canon, sig, arch, model ) )
+        return fail( FFTM_MI_INVALID_DEVICE, "fftm_mi_hardware_signature: no device" );
+    if ( !buf || len < sig.size() + 1 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_hardware_signature: buffer too small" );
+    std::memcpy( buf, sig.c_str(), sig.size() + 1 );
+    return ok();
+}
+
+int fftm_mi_ffts_load_tuning( fftm_mi_ffts h, const char *cache_path, int32_t *applied )
+{
+    if ( !h || !cache_path )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_load_tuning: null argument" );
+    if ( applied )
+        *applied = 0;
+    cache_map  m;
+    const int  rc = cache_load( cache_path, m );
+    if ( rc == 1 )
+        return ok(); // no cache yet: defaults stay
+    if ( rc < 0 )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( "fftm_mi_ffts_load_tuning: " ) + cache_path + ":" +
+                                                std::to_string( -rc ) + " is not KEY=VALUE" );
+    std::string canon, sig, arch, model;
+    if ( !hardware_signature( canon, sig, arch, model ) )
+        return fail( FFTM_MI_INVALID_DEVICE, "fftm_mi_ffts_load_tuning: no device" );
+    if ( m["FFTM_AUTOTUNE_HARDWARE_SIGNATURE"] != sig )
+        return ok(); // measured on other hardware: not applied
+    const std::string prob = problem_key( *h );
+    int               n    = 0;
+    for ( int bw = 0; bw < 2; ++bw )
+    {
+        std::vector<pass_rt> &v = bw ? h->bwd : h->fwd;
+        for ( size_t i = 0; i < v.size(); ++i )
+        {
+            auto it = m.find( pass_key( prob, bw, i ) );
+            if ( it == m.end() )
+                continue;
+            const int var = std::atoi( it->second.c_str() );
+            if ( var >= 0 && var < NUM_VARIANTS )
+            {
+                v[i].var = var;
+                ++n;
+            }
+        }
+    }
+    if ( applied )
+        *applied = n;
+    return ok();
+}
+
+int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_out, int32_t warmup,
+                           int32_t iterations, const char *cache_path )
+{
+    if ( !h || !real_in || !spectrum_out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_autotune: null argument" );
+    if ( h->work_size > 0 && !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_autotune: a work area is required" );
+    if ( iterations < 1 || warmup < 0 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_autotune: iterations >= 1, warmup >= 0" );
+    // the real output of the backward passes needs its own buffer: same bytes as the input
+    int64_t nreal = 1, ncplx = 1;
+    const int dim = h->layout == FFTM_MI_LAYOUT_3D ? 3 : 4;
+    for ( int k = 0; k < dim; ++k )
+    {
+        nreal *= h->n[k];
+        ncplx *= k == dim - 1 ? h->n[k] / 2 + 1 : h->n[k];
+    }
+    const size_t rbytes = (size_t)nreal * ( h->prec == FFTM_MI_F32 ? 4 : 8 );
+    const size_t cbytes = (size_t)ncplx * ( h->prec == FFTM_MI_F32 ? 8 : 16 );
+    void               *real_out = nullptr, *ref = nullptr;
+    unsigned long long *dcount   = nullptr;
+    hipEvent_t          e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if ( real_out )
+            (void)hipFree( real_out );
+        if ( ref )
+            (void)hipFree( ref );
+        if ( dcount )
+            (void)hipFree( dcount );
+        if ( e0 )
+            (void)hipEventDestroy( e0 );
+        if ( e1 )
+            (void)hipEventDestroy( e1 );
+    };
+    if ( hipMalloc( &real_out, rbytes ) != hipSuccess || hipMalloc( &ref, std::max( rbytes, cbytes ) ) != hipSuccess ||
+         hipMalloc( &dcount, sizeof( unsigned long long ) ) != hipSuccess || hipEventCreate( &e0 ) != hipSuccess ||
+         hipEventCreate( &e1 ) != hipSuccess )
+    {
+        cleanup();
+        return fail( FFTM_MI_ALLOC_FAILED, "fftm_mi_ffts_autotune: scratch allocation failed" );
+    }
+    const std::string prob = problem_key( *h );
+    cache_map         rec;
+    int               rc = 0;
+    for ( int bw = 0; bw < 2 && rc == 0; ++bw )
+    {
+        std::vector<pass_rt> &v   = bw ? h->bwd : h->fwd;
+        const void           *in  = bw ? (const void *)spectrum_out : real_in;
+        void                 *out = bw ? real_out : spectrum_out;
+        for ( size_t i = 0; i < v.size() && rc == 0; ++i )
+        {
+            // the pass writes into: out (its dst) -- compare the whole destination buffer
+            const pass_rt &cur     = v[i];
+            void          *dstbuf  = cur.dst == BUF_OUT ? out : ( cur.dst == BUF_WORK ? h->work : const_cast<void *>( in ) );
+            const size_t   dbytes  = cur.dst == BUF_OUT ? ( bw ? rbytes : cbytes ) : cbytes;
+            std::vector<pass_rt> prefix( v.begin(), v.begin() + (long)i );
+            std::vector<pass_rt> one( 1, cur );
+            double               best_ms = 1e300;
+            int                  best    = cur.var;
+            const int            nvar    = ( cur.plane || cur.generic ) ? 1 : NUM_VARIANTS;
+            for ( int var = 0; var < nvar && rc == 0; ++var )
+            {
+                one[0].var = var;
+                // regenerate the pass input, run once, check against variant 0
+                if ( bw )
+                    rc = run_passes( h->fwd, h->tb, h->prec, real_in, spectrum_out, h->work, h->stream, false, false, "autotune" );
+                if ( rc == 0 )
+                    rc = run_passes( prefix, h->tb, h->prec, in, out, h->work, h->stream, false, false, "autotune" );
+                if ( rc == 0 )
+                    rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, true, "autotune" );
+                if ( rc != 0 )
+                    break;
+                bool valid = true;
+                if ( var == 0 )
+                    (void)hipMemcpy( ref, dstbuf, dbytes, hipMemcpyDeviceToDevice );
+                else
+                {
+                    (void)hipMemset( dcount, 0, sizeof( unsigned long long ) );
+                    hipLaunchKernelGGL( k_count_diff, dim3( 1024 ), dim3( 256 ), 0, h->stream,
+                                        static_cast<const uint32_t *>( ref ), static_cast<const uint32_t *>( dstbuf ),
+                                        (int64_t)( dbytes / 4 ), dcount );
+                    unsigned long long diff = 0;
+                    (void)hipMemcpy( &diff, dcount, sizeof( diff ), hipMemcpyDeviceToHost );
+                    valid = diff == 0;
+                }
+                // timing: warmup, then `iterations` single launches, median
+                std::vector<float> ms;
+                for ( int w = 0; w < warmup && rc == 0; ++w )
+                    rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, false, "autotune" );
+                for ( int it = 0; it < iterations && rc == 0; ++it )
+                {
+                    (void)hipEventRecord( e0, h->stream );
+                    rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, false, "autotune" );
+                    (void)hipEventRecord( e1, h->stream );
+                    (void)hipEventSynchronize( e1 );
+                    float t = 0.f;
+                    (void)hipEventElapsedTime( &t, e0, e1 );
+                    ms.push_back( t );
+                }
+                if ( rc != 0 )
+                    break;
+                std::sort( ms.begin(), ms.end() );
+                const double med = ms.size() % 2 ? ms[ms.size() / 2] : 0.5 * ( ms[ms.size() / 2 - 1] + ms[ms.size() / 2] );
+                char         pfx[32];
+                std::snprintf( pfx, sizeof( pfx ), "_CANDIDATE_%03d_", var );
+                const std::string k = pass_key( prob, bw, i ) + pfx;
+                std::ostringstream mo;
+                mo << std::setprecision( 9 ) << med;
+                rec[k + "VALID"]     = valid ? "1" : "0";
+                rec[k + "MEDIAN_MS"] = valid ? mo.str() : "nan";
+                if ( valid && med < best_ms )
+                {
+                    best_ms = med;
+                    best    = var;
+                }
+            }
+            if ( rc == 0 )
+            {
+                v[i].var                        = best;
+                rec[pass_key( prob, bw, i )]    = std::to_string( best );
+                std::ostringstream mo;
+                mo << std::setprecision( 9 ) << best_ms;
+                rec[pass_key( prob, bw, i ) + "_WINNER_MEDIAN_MS"] = mo.str();
+            }
+        }
+    }
+    cleanup();
+    if ( rc != 0 )
+        return rc;
+    if ( cache_path && *cache_path )
+    {
+        cache_map m;
+        const int lr = cache_load( cache_path, m );
+        if ( lr < 0 )
+            m.clear(); // a malformed cache is replaced
+        std::string canon, sig, arch, model;
+        if ( !hardware_signature( canon, sig, arch, model ) )
+            return fail( FFTM_MI_INVALID_DEVICE, "fftm_mi_ffts_autotune: no device" );
+        if ( m.count( "FFTM_AUTOTUNE_HARDWARE_SIGNATURE" ) && m["FFTM_AUTOTUNE_HARDWARE_SIGNATURE"] != sig )
+            m.clear(); // measured elsewhere: start a new cache for this hardware
+        m["FFTM_AUTOTUNE_SCHEMA"]                     = "2";
+        m["FFTM_AUTOTUNE_SOURCE"]                     = "cpp-measured-v2";
+        m["FFTM_AUTOTUNE_MEASURED"]                   = "1";
+        m["FFTM_AUTOTUNE_MEASURE_WARMUP"]             = std::to_string( warmup );
+        m["FFTM_AUTOTUNE_MEASURE_ITERATIONS"]         = std::to_string( iterations );
+        m["FFTM_AUTOTUNE_HARDWARE_SIGNATURE_VERSION"] = "2";
+        m["FFTM_AUTOTUNE_HARDWARE_SIGNATURE"]         = sig;
+        m["FFTM_AUTOTUNE_HARDWARE_CANONICAL"]         = canon;
+        m["FFTM_AUTOTUNE_GPU_ARCHITECTURES"]          = arch;
+        m["FFTM_AUTOTUNE_GPU_MODELS"]                 = model;
+        for ( const auto &kv : rec )
+            m[kv.first] = kv.second;
+        if ( !cache_save( cache_path, m ) )
+            return fail( FFTM_MI_INVALID_VALUE, std::string( "fftm_mi_ffts_autotune: cannot write " ) + cache_path );
+    }
     return ok();
 }
 

@@ -58,6 +58,8 @@ hipError_t launch_n( const pass_params<T> &p, int kind, bool inv, bool loadb, in
         case 1: return launch_nv<T, N, 1>( p, kind, inv, loadb, s );
         case 2: return launch_nv<T, N, 2>( p, kind, inv, loadb, s );
         case 3: return launch_nv<T, N, 3>( p, kind, inv, loadb, s );
+        case 4: return launch_nv<T, N, 4>( p, kind, inv, loadb, s );
+        case 5: return launch_nv<T, N, 5>( p, kind, inv, loadb, s );
         default: break;
         }
     }
@@ -80,6 +82,8 @@ int tile_cols_n( bool loadb, int var )
         case 1: return tile_cols_nv<T, N, 1>( loadb );
         case 2: return tile_cols_nv<T, N, 2>( loadb );
         case 3: return tile_cols_nv<T, N, 3>( loadb );
+        case 4: return tile_cols_nv<T, N, 4>( loadb );
+        case 5: return tile_cols_nv<T, N, 5>( loadb );
         default: break;
         }
     }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mi_pass.hpp"
+#include "mi_plane.hpp"
 
 namespace fftm_mi
 {
@@ -22,6 +23,11 @@ int fixed_tile_cols_f64( int N, bool loadb, int var );
 
 hipError_t launch_pass_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s );
 hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s );
+// plane passes (real axis + one complex axis in LDS), square planes 16..128
+bool       plane_supported( int64_t na, int64_t nb, int prec );
+hipError_t launch_plane_f32( const plane_params<float> &p, int na, int nb, bool fwd, hipStream_t s );
+hipError_t launch_plane_f64( const plane_params<double> &p, int na, int nb, bool fwd, hipStream_t s );
+
 hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
                                hipStream_t s );
 hipError_t launch_generic_f64( const pass_params<double> &p, const radix_list &rl, int N, int cols, int kind, bool inv,

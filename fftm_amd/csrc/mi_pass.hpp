@@ -137,6 +137,7 @@ struct geom
 };
 
 constexpr int lds_limit_bytes() { return 65536; }
+constexpr int lds_limit_wide_bytes() { return 163840; }
 
 constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 
@@ -144,7 +145,9 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 //   1: half the columns per workgroup (smaller LDS tile, more workgroups per CU)
 //   2: twice the threads per column (fewer elements per thread, larger workgroup)
 //   3: both (same workgroup size, half the tile)
-constexpr int NUM_VARIANTS = 4;
+//   4: twice the columns (256-byte row segments; LDS up to 160 KB on gfx950)
+//   5: twice the columns and twice the threads per column (1024-thread workgroups)
+constexpr int NUM_VARIANTS = 6;
 
 constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 {
@@ -190,6 +193,17 @@ constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
         g.tw /= 2;
     if ( ( var == 2 || var == 3 ) && g.tpc * 2 <= hi && g.tpc * 2 * g.tw <= 1024 )
         g.tpc *= 2;
+    if ( var == 4 || var == 5 )
+    {
+        const int pa    = loadb ? N : geom_pa( N, g.pad );
+        const int tw2   = g.tw * 2;
+        const int tpc2  = var == 5 && g.tpc * 2 <= hi ? g.tpc * 2 : g.tpc;
+        if ( pa * tw2 * csize <= lds_limit_wide_bytes() && tw2 * tpc2 <= 1024 )
+        {
+            g.tw  = tw2;
+            g.tpc = tpc2;
+        }
+    }
     return g;
 }
 
@@ -289,7 +303,8 @@ struct pass_kernel
     static constexpr int   PB  = TW;
     static constexpr int   LDS = LOADB ? N * PB : TW * PA;
     static constexpr int   CPT = KIND == KIND_C2C ? 1 : 2;
-    static_assert( LDS * (int)sizeof( C ) <= lds_limit_bytes(), "LDS tile too large" );
+    static_assert( LDS * (int)sizeof( C ) <= ( VAR >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
+                   "LDS tile too large" );
 
     __device__ static __forceinline__ int L( int n, int c )
     {

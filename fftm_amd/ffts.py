@@ -412,6 +412,45 @@ class ffts:
             else:
                 self._backward_4d_staged(inp, out)
 
+    # ------------------------------------------------------------- tuning
+    def autotune(self, warmup: int = 3, iterations: int = 10, cache_path: Optional[str] = None):
+        """Measure every tile-shape variant of every pass and keep the fastest valid one
+        (reference autotune protocol; fused execution only).  Returns the chosen variants
+        as {(backward, index): variant}."""
+        import torch
+        self._need_handle("autotune")
+        u = torch.rand(self.real_shape(), dtype=self.real, device="cuda")
+        X = self.complex_array()
+        _lib.check(_lib.lib().fftm_mi_ffts_autotune(self._handle, device_ptr(u), device_ptr(X), int(warmup),
+                                                    int(iterations),
+                                                    None if cache_path is None else cache_path.encode()),
+                   "fftm_mi_ffts_autotune")
+        return self.pass_variants()
+
+    def load_tuning(self, cache_path: str) -> int:
+        self._need_handle("load_tuning")
+        n = ctypes.c_int32()
+        _lib.check(_lib.lib().fftm_mi_ffts_load_tuning(self._handle, cache_path.encode(), ctypes.byref(n)),
+                   "fftm_mi_ffts_load_tuning")
+        return int(n.value)
+
+    def pass_variants(self):
+        self._need_handle("pass_variants")
+        nf, nb = self.num_passes()
+        out = {}
+        for bw, n in ((0, nf), (1, nb)):
+            for i in range(n):
+                v = ctypes.c_int32()
+                _lib.check(_lib.lib().fftm_mi_ffts_get_pass_variant(self._handle, bw, i, ctypes.byref(v)),
+                           "fftm_mi_ffts_get_pass_variant")
+                out[(bw, i)] = int(v.value)
+        return out
+
+    def set_pass_variant(self, backward: int, index: int, variant: int):
+        self._need_handle("set_pass_variant")
+        _lib.check(_lib.lib().fftm_mi_ffts_set_pass_variant(self._handle, int(backward), int(index), int(variant)),
+                   "fftm_mi_ffts_set_pass_variant")
+
     # -------------------------------------------- adjoints / spectral operators
     # Spectrum inner product <X, Y> = Re sum conj(X) Y.  With w_k = 1 on the DC and
     # Nyquist planes of the real axis and 2 elsewhere: R2C^T(G) = C2R(G / w) and

@@ -58,6 +58,9 @@ extern "C" {
  * input-sized complex buffer.  With this flag the input is used as scratch
  * (hipFFT's documented out-of-place C2R behaviour) and the work size is 0. */
 #define FFTM_MI_FLAG_OVERWRITE_INPUT 1
+/* ffts handles: keep the real axis and its partner axis in separate passes instead of
+ * one plane pass (used to measure the fusion; results are the same within rounding). */
+#define FFTM_MI_FLAG_NO_PLANE_FUSION 2
 
 /*
  * Plan descriptor: the fields of fftm::wrap::fft_plan_descriptor
@@ -165,9 +168,11 @@ int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *b
  * passed; the pass reads and writes exactly what it does inside that call.  Used to
  * time individual kernels; it has no counterpart in the reference. */
 int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
-/* Tile-shape variant (0..3) of one pass's kernel: 0 default, 1 half the columns per
- * workgroup, 2 twice the threads per column, 3 both.  Results are bitwise identical
- * across variants; only the speed differs.  Set by the tuner or by hand. */
+/* Tile-shape variant (0..5) of one pass's kernel: 0 default, 1 half the columns per
+ * workgroup, 2 twice the threads per column, 3 both, 4 twice the columns (256-byte row
+ * segments, up to 160 KB of LDS), 5 twice the columns and the threads per column.
+ * Results are bitwise identical across variants; only the speed differs.  Set by the
+ * tuner or by hand. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
 int fftm_mi_ffts_get_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t *variant );
 /* Algorithmic HBM bytes one pass moves (reads + writes of its input and output). */
@@ -204,6 +209,26 @@ int fftm_mi_ffts_spectral_multiply( fftm_mi_ffts h, const void *in, void *out, c
  * coefficients, G = dL/dZ for Z = H X), accumulated in FP64, written as T[nshells]. */
 int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, const void *spectrum, void *grad,
                                  int32_t nshells, int32_t sync );
+
+/*
+ * Tuning of the tile-shape variants with the reference's autotune cache protocol
+ * (source/fftm_autotune.hpp:126-169, fftm_autotune_measure.hpp:293-400,
+ * detail/fftm_autotune_hardware.inc:87-98, 285-337): for every pass of the handle and
+ * every variant, regenerate the pass input, check the output is bitwise equal to
+ * variant 0, time `iterations` launches after `warmup` (median), keep the fastest valid
+ * one.  real_in / spectrum_out are scratch buffers of the forward's shapes (overwritten).
+ * With cache_path the per-candidate records and winners are merged into a KEY=VALUE
+ * cache ("# FFTM C++ autotune cache", tmp + rename) keyed by problem and by the v2
+ * hardware signature (FNV-1a 64 of the canonical device string).
+ * load_tuning applies a cache's winners when its signature matches this device; a
+ * handle created while FFTM_MI_AUTOTUNE_CACHE names a cache file loads it.
+ */
+int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_out, int32_t warmup,
+                           int32_t iterations, const char *cache_path );
+int fftm_mi_ffts_load_tuning( fftm_mi_ffts h, const char *cache_path, int32_t *applied );
+int fftm_mi_hardware_signature( char *buf, size_t len );
+/* FNV-1a 64 of a string as 16 lowercase hex digits (host only). */
+int fftm_mi_fnv1a64( const char *text, char *hex17 );
 
 /* Last error message of the calling thread ("" when none). */
 const char *fftm_mi_last_error( void );

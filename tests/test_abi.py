@@ -157,3 +157,45 @@ def test_python_mirror_logic_errors_without_gpu():
 
 def test_version():
     assert _lib().lib().fftm_mi_version() >= 100
+
+
+def fnv1a64_py(text: bytes) -> str:
+    """detail/fftm_autotune_hardware.inc:87-98 restated: offset basis 14695981039346656037, prime 1099511628211."""
+    h = 14695981039346656037
+    for c in text:
+        h ^= c
+        h = (h * 1099511628211) & ((1 << 64) - 1)
+    return f"{h:016x}"
+
+
+@pytest.mark.parametrize("text,expected", [(b"", "cbf29ce484222325"), (b"a", "af63dc4c8601ec8c"),
+                                           (b"foobar", "85944171f73967e8")])
+def test_fnv1a64_published_vectors(text, expected):
+    """FNV-1a 64 test vectors (Fowler/Noll/Vo reference suite)."""
+    L = _lib()
+    buf = ctypes.create_string_buffer(17)
+    assert L.lib().fftm_mi_fnv1a64(text, buf) == 0
+    assert buf.value.decode() == expected == fnv1a64_py(text)
+
+
+def test_fnv1a64_canonical_string():
+    L = _lib()
+    canon = b"version=2;device_classes=gfx950;device_models=AMD_Instinct_MI355X;compute_units=256"
+    buf = ctypes.create_string_buffer(17)
+    assert L.lib().fftm_mi_fnv1a64(canon, buf) == 0
+    assert buf.value.decode() == fnv1a64_py(canon)
+
+
+def test_hardware_signature_needs_device():
+    L = _lib()
+    buf = ctypes.create_string_buffer(64)
+    rc = L.lib().fftm_mi_hardware_signature(buf, 64)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        assert rc == 11
+    else:
+        assert rc == 0 and buf.value.startswith(b"v2:")
