@@ -61,7 +61,7 @@ def _traffic_summary(kernel_key):
         return None
 
 
-def run_gpu(dims, strategy, steps, warmup, check):
+def run_gpu(dims, strategy, steps, warmup, check, tune=True):
     import torch
     import fftm_amd
     from fftm_amd import _lib
@@ -75,6 +75,9 @@ def run_gpu(dims, strategy, steps, warmup, check):
     f.set_stream(stream)
     h = f._handle
     L = _lib.lib()
+    # plan-time tuning, outside the timed region (the reference's autotune protocol:
+    # every tile-shape variant of every pass timed, fastest bitwise-valid one kept)
+    variants = f.autotune(warmup=2, iterations=7) if tune else f.pass_variants()
 
     # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor), resident in HBM
     import oracle
@@ -126,7 +129,7 @@ def run_gpu(dims, strategy, steps, warmup, check):
     api_pair_s = (time.perf_counter() - t2) / steps
 
     res = {"pair_s": pair_s, "api_pair_s": api_pair_s, "per_pass_ms": per_pass_ms, "pass_bytes": pbytes,
-           "passes": passes}
+           "passes": passes, "variants": {f"{'bwd' if b else 'fwd'}{i}": v for (b, i), v in variants.items()}}
     if check:
         pair()
         torch.cuda.synchronize()
@@ -181,6 +184,7 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-4d", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes (skip plan-time tuning)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -211,7 +215,8 @@ def main():
     n = args.size
     dims = (n, n, n)
     N = n ** 3
-    r3 = run_gpu(dims, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check)
+    r3 = run_gpu(dims, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check,
+                 not args.no_tune)
     pair_s = r3["pair_s"]
     gflops = _flops_pair(N) / pair_s / 1e9
     balg = _b_alg(dims, 4)
@@ -242,6 +247,7 @@ def main():
         "b_alg_bytes": balg,
         "api_ms_per_pair": round(r3["api_pair_s"] * 1e3, 4),
         "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r3["per_pass_ms"].items()},
+        "pass_variants": r3["variants"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "avg_ms": round(dom_ms, 4)},
@@ -251,7 +257,8 @@ def main():
                                                            "roundtrip_rms", "roundtrip_max")}
     if not args.no_4d:
         d4 = (64, 64, 64, 64)
-        r4 = run_gpu(d4, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check)
+        r4 = run_gpu(d4, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check,
+                     not args.no_tune)
         N4 = 64 ** 4
         out["secondary_64^4"] = {
             "workload": "ffts 4D pencil layout R2C+C2R pair 64^4 float32 (real [w][z][y][x] -> [y][z][w/2+1][x])",
@@ -259,6 +266,7 @@ def main():
             "ms_per_pair": round(r4["pair_s"] * 1e3, 4),
             "hbm_gbps_alg": round(_b_alg(d4, 4) / r4["pair_s"] / 1e9, 1),
             "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r4["per_pass_ms"].items()},
+            "pass_variants": r4["variants"],
         }
         if not args.no_check:
             out["secondary_64^4"]["errors_vs_fp64_oracle"] = {

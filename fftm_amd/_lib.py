@@ -29,6 +29,7 @@ NOT_SUPPORTED = 16
 
 R2C, C2R, C2CF, C2CB = 1, 2, 3, 4
 F32, F64 = 0, 1
+F32_H16 = 2  # FP32 transforms, FP16-stored intermediate spectra
 FLAG_OVERWRITE_INPUT = 1
 
 LAYOUT_3D = 0
@@ -51,6 +52,15 @@ class FftmMiError(RuntimeError):
     def __init__(self, status: int, where: str, message: str):
         self.status = status
         super().__init__(f"{where}: {_STATUS_NAMES.get(status, 'unknown')} ({status}): {message}")
+
+
+OP_POISSON, OP_GAUSSIAN, OP_SHELL = 1, 2, 3
+
+
+class SpectralOp(ctypes.Structure):
+    """fftm_mi_spectral_op."""
+    _fields_ = [("kind", ctypes.c_int32), ("nshells", ctypes.c_int32), ("scale", ctypes.c_double),
+                ("sigma", ctypes.c_double), ("coeffs", ctypes.c_void_p)]
 
 
 class PlanDesc(ctypes.Structure):
@@ -99,6 +109,8 @@ SIGNATURES = {
     "fftm_mi_ffts_c2r_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
     "fftm_mi_ffts_spectral_multiply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32]),
     "fftm_mi_ffts_shell_gradient": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32]),
+    "fftm_mi_ffts_spectral_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32]),
+    "fftm_mi_ffts_filter": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32]),
     "fftm_mi_ffts_autotune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, ctypes.c_char_p]),
     "fftm_mi_ffts_load_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _i32p]),
     "fftm_mi_hardware_signature": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),

@@ -39,20 +39,24 @@ __device__ __forceinline__ cpx<T> sub( cpx<T> a, cpx<T> b )
     return mk<T>( a.x - b.x, a.y - b.y );
 }
 
+// Fused multiply-add in the precision of T (one rounding; v_fma_f32 / v_fma_f64).
+__device__ __forceinline__ float fmat( float a, float b, float c ) { return __builtin_fmaf( a, b, c ); }
+__device__ __forceinline__ double fmat( double a, double b, double c ) { return __builtin_fma( a, b, c ); }
+
 // Complex products use explicit fused multiply-adds (the library is compiled with
 // -ffp-contract=off), so every kernel instantiation rounds identically whatever the
 // unrolling: tile-shape variants stay bitwise equal.
 template <typename T>
 __device__ __forceinline__ cpx<T> mul( cpx<T> a, cpx<T> b )
 {
-    return mk<T>( __builtin_fma( a.x, b.x, -( a.y * b.y ) ), __builtin_fma( a.x, b.y, a.y * b.x ) );
+    return mk<T>( fmat( a.x, b.x, -( a.y * b.y ) ), fmat( a.x, b.y, a.y * b.x ) );
 }
 
 // a * conj(b)
 template <typename T>
 __device__ __forceinline__ cpx<T> mulc( cpx<T> a, cpx<T> b )
 {
-    return mk<T>( __builtin_fma( a.x, b.x, a.y * b.y ), __builtin_fma( a.y, b.x, -( a.x * b.y ) ) );
+    return mk<T>( fmat( a.x, b.x, a.y * b.y ), fmat( a.y, b.x, -( a.x * b.y ) ) );
 }
 
 template <typename T>

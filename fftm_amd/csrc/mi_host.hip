@@ -51,6 +51,7 @@ enum : int
 struct bdim
 {
     int64_t n, is, os;
+    int64_t xs = 0, ys = 0; // exponent strides of FP16-stored input / output lines
 };
 
 // One HBM pass: a batch of 1D transforms along one axis.
@@ -71,6 +72,13 @@ struct pass_rt
     int        wmode    = 0; // real-axis bin weights (adjoints), see pass_params::wmode
     int        var      = 0; // tile-shape variant of the fixed-length kernel
     double     scale    = 1.0;
+    // FP16-stored spectra (mixed precision): IO_H16_IN / IO_H16_OUT bits, exponent
+    // strides (input: batch dims and transform axis; output: batch dims) and the byte
+    // offsets of the exponent arrays in the work area
+    bool       filter   = false; // fused filter pass (forward C2C, x H(k), inverse C2C)
+    int        iom      = 0;
+    int64_t    xsn = 0, xs0 = 0, xs1 = 0, xs2 = 0, ys0 = 0, ys1 = 0, ys2 = 0;
+    int64_t    xoff = 0, yoff = 0;
     // plane pass (the real axis A of length N and axis B of length NB together)
     bool       plane    = false;
     int64_t    NB       = 0;
@@ -162,7 +170,8 @@ int set_batch( pass_rt &p, std::vector<bdim> dims )
     std::vector<bdim> m;
     for ( const bdim &d : v )
     {
-        if ( !m.empty() && m.back().n * m.back().is == d.is && m.back().n * m.back().os == d.os )
+        if ( !m.empty() && m.back().n * m.back().is == d.is && m.back().n * m.back().os == d.os &&
+             m.back().n * m.back().xs == d.xs && m.back().n * m.back().ys == d.ys )
             m.back().n *= d.n;
         else
             m.push_back( d );
@@ -171,6 +180,12 @@ int set_batch( pass_rt &p, std::vector<bdim> dims )
         return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi: pass needs more than 3 batch dimensions" );
     while ( m.size() < 3 )
         m.push_back( bdim{ 1, 0, 0 } );
+    p.xs0 = m[0].xs;
+    p.xs1 = m[1].xs;
+    p.xs2 = m[2].xs;
+    p.ys0 = m[0].ys;
+    p.ys1 = m[1].ys;
+    p.ys2 = m[2].ys;
     p.ncols = m[0].n * m[1].n * m[2].n;
     p.d0    = m[0].n;
     p.d1    = m[1].n;
@@ -269,12 +284,33 @@ hipError_t launch_plane_one( const pass_rt &r, const tables &tb, const void *src
         return launch_plane_f64( p, (int)r.N, (int)r.NB, fwd, s );
 }
 
+// Paired accesses (two consecutive elements per load / store) need unit element stride,
+// a 16-byte aligned base and an even element offset for every line start.
+bool even_line_offsets( const pass_rt &r, int64_t s0, int64_t s1, int64_t s2 )
+{
+    const int64_t d2 = ( r.ncols + r.d0 * r.d1 - 1 ) / ( r.d0 * r.d1 );
+    return ( r.d0 <= 1 || s0 % 2 == 0 ) && ( r.d1 <= 1 || s1 % 2 == 0 ) && ( d2 <= 1 || s2 % 2 == 0 );
+}
+
+int32_t pair_access_bits( const pass_rt &r, const void *src, const void *dst )
+{
+    int32_t bits = 0;
+    if ( !r.loadb && r.ies == 1 && ( reinterpret_cast<uintptr_t>( src ) & 15 ) == 0 &&
+         even_line_offsets( r, r.is0, r.is1, r.is2 ) )
+        bits |= 1;
+    if ( !r.storeb && r.oes == 1 && ( reinterpret_cast<uintptr_t>( dst ) & 15 ) == 0 &&
+         even_line_offsets( r, r.os0, r.os1, r.os2 ) )
+        bits |= 2;
+    return bits;
+}
+
 template <typename T>
-hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void *dst, hipStream_t s )
+hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void *dst, hipStream_t s,
+                       void *work = nullptr, const void *filter = nullptr )
 {
     if ( r.plane )
         return launch_plane_one<T>( r, tb, src, dst, s );
-    pass_params<T> p;
+    pass_params<T> p{};
     p.in      = src;
     p.out     = dst;
     p.tw      = static_cast<const cpx<T> *>( tb.by_n.at( r.N ) );
@@ -291,17 +327,42 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.os2     = r.os2;
     p.store_b = r.storeb ? 1 : 0;
     p.wmode   = r.wmode;
+    p.vec     = pair_access_bits( r, src, dst );
     p.scale   = (T)r.scale;
     if constexpr ( std::is_same<T, float>::value )
+    {
+        if ( r.iom )
+        {
+            unsigned char *w = static_cast<unsigned char *>( work );
+            p.xin            = ( r.iom & IO_H16_IN ) ? reinterpret_cast<const int8_t *>( w + r.xoff ) : nullptr;
+            p.xout           = ( r.iom & IO_H16_OUT ) ? reinterpret_cast<int8_t *>( w + r.yoff ) : nullptr;
+            p.xsn            = r.xsn;
+            p.xs0            = r.xs0;
+            p.xs1            = r.xs1;
+            p.xs2            = r.xs2;
+            p.ys0            = r.ys0;
+            p.ys1            = r.ys1;
+            p.ys2            = r.ys2;
+            p.vec            = 0;
+            return launch_pass_mixed_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, r.iom, s );
+        }
+        if ( r.filter )
+            return launch_filter_f32( p, *static_cast<const filter_params<float> *>( filter ), (int)r.N, r.loadb, s );
         return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
                          : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
+    }
     else
+    {
+        if ( r.filter )
+            return launch_filter_f64( p, *static_cast<const filter_params<double> *>( filter ), (int)r.N, r.loadb, s );
         return r.generic ? launch_generic_f64( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
                          : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
+    }
 }
 
 int run_passes( const std::vector<pass_rt> &passes, const tables &tb, int prec, const void *in, void *out,
-                void *work, hipStream_t s, bool flip_c2c, bool sync, const char *what, int real_wmode = 0 )
+                void *work, hipStream_t s, bool flip_c2c, bool sync, const char *what, int real_wmode = 0,
+                const void *filter = nullptr )
 {
     for ( const pass_rt &r0 : passes )
     {
@@ -312,8 +373,8 @@ int run_passes( const std::vector<pass_rt> &passes, const tables &tb, int prec, 
             r.wmode = real_wmode;
         const void *src = r.src == BUF_IN ? in : ( r.src == BUF_OUT ? (const void *)out : (const void *)work );
         void       *dst = r.dst == BUF_OUT ? out : ( r.dst == BUF_WORK ? work : const_cast<void *>( in ) );
-        hipError_t  e   = prec == FFTM_MI_F32 ? launch_one<float>( r, tb, src, dst, s )
-                                              : launch_one<double>( r, tb, src, dst, s );
+        hipError_t  e   = prec == FFTM_MI_F32 ? launch_one<float>( r, tb, src, dst, s, work, filter )
+                                              : launch_one<double>( r, tb, src, dst, s, work, filter );
         if ( e != hipSuccess )
             return fail( FFTM_MI_EXEC_FAILED,
                          std::string( "fftm_mi: " ) + what + ": kernel launch failed: " + hipGetErrorString( e ) );
@@ -332,7 +393,14 @@ int prepare_passes( std::vector<pass_rt> &passes, int prec, tables *tb )
 {
     for ( pass_rt &p : passes )
     {
-        if ( !p.plane )
+        if ( p.iom )
+        {
+            if ( !mixed_supported( p.N ) )
+                return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi: FP16-stored spectra need power-of-two lengths 128..2048, got " +
+                                                        std::to_string( p.N ) );
+            p.generic = false;
+        }
+        else if ( !p.plane )
         {
             const int rc = configure_kernel( p, prec );
             if ( rc != FFTM_MI_SUCCESS )
@@ -587,6 +655,49 @@ pass_rt axis_pass( int kind, bool inv, int ax, int dim, const int64_t *ext, cons
     return p;
 }
 
+// Exponent strides of the lines a pass writes (FP16-stored spectra): line ids are dense
+// over the pass's batch axes, in the order of the output data strides.
+void dense_line_strides( int ax, int dim, const int64_t *ext, const int64_t *so, int64_t *ys )
+{
+    int order[4], n = 0;
+    for ( int k = 0; k < dim; ++k )
+        if ( k != ax )
+            order[n++] = k;
+    std::sort( order, order + n, [&]( int a, int b ) { return so[a] < so[b]; } );
+    int64_t acc = 1;
+    for ( int i = 0; i < n; ++i )
+    {
+        ys[order[i]] = acc;
+        acc *= ext[order[i]];
+    }
+    ys[ax] = 0;
+}
+
+// axis_pass with FP16 storage: xs (per logical axis) = exponent strides of the input
+// lines or null for FP32 input; ys = exponent strides of the output lines or null
+pass_rt axis_pass_h16( int kind, bool inv, int ax, int dim, const int64_t *ext, const int64_t *si, const int64_t *so,
+                       int src, int dst, int64_t N, const int64_t *xs, const int64_t *ys, int &rc )
+{
+    pass_rt p;
+    p.kind = kind;
+    p.inv  = inv;
+    p.N    = N;
+    p.src  = src;
+    p.dst  = dst;
+    p.ies  = si[ax];
+    p.oes  = so[ax];
+    p.iom  = ( xs ? IO_H16_IN : 0 ) | ( ys ? IO_H16_OUT : 0 );
+    p.xsn  = xs ? xs[ax] : 0;
+    std::vector<bdim> v;
+    for ( int k = 0; k < dim; ++k )
+        if ( k != ax )
+            v.push_back( bdim{ ext[k], si[k], so[k], xs ? xs[k] : 0, ys ? ys[k] : 0 } );
+    rc = set_batch( p, v );
+    // strided lines: twice the columns per workgroup (FP16 rows of 128 bytes), split exchange
+    p.var = p.loadb ? 7 : 0;
+    return p;
+}
+
 struct ffts_layout
 {
     int     dim;
@@ -764,6 +875,62 @@ int build_ffts_passes( const ffts_layout &L, bool overwrite, std::vector<pass_rt
     return rc;
 }
 
+// 3D transform with FP16-stored intermediate spectra: the real-axis pass writes the
+// half spectrum in FP16 to the work area, the middle pass keeps it there, the last
+// forward pass writes the FP32 spectrum; backward mirrors it.  Work area:
+// [FP16 spectrum: 4 bytes per bin][exponents E0][exponents E1], 256-byte aligned.
+int build_ffts_passes_h16( const ffts_layout &L, std::vector<pass_rt> &fwd, std::vector<pass_rt> &bwd,
+                           size_t &work_bytes )
+{
+    if ( L.dim != 3 )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi: FP16-stored spectra are implemented for the 3D layout" );
+    const int64_t n0 = L.n[0], n1 = L.n[1], n2 = L.n[2], nzh = n2 / 2 + 1;
+    const int64_t extc[3] = { n0, n1, nzh };
+    auto          align   = []( size_t b ) { return ( b + 255 ) & ~(size_t)255; };
+    const size_t  hbytes  = align( (size_t)( n0 * n1 * nzh ) * 4 );
+    const size_t  e0bytes = align( (size_t)std::max( n0 * n1, n1 * nzh ) );
+    const size_t  e1bytes = align( (size_t)( n0 * nzh ) );
+    const int64_t E0 = (int64_t)hbytes, E1 = (int64_t)( hbytes + e0bytes );
+    work_bytes       = hbytes + e0bytes + e1bytes;
+    int64_t yP0[3], yP1[3], yQ0[3], yQ1[3];
+    dense_line_strides( 2, 3, L.n, L.sc, yP0 );
+    dense_line_strides( 1, 3, extc, L.sc, yP1 );
+    dense_line_strides( 0, 3, extc, L.sc, yQ0 );
+    dense_line_strides( 1, 3, extc, L.sc, yQ1 );
+    int rc = 0;
+    fwd.clear();
+    bwd.clear();
+    pass_rt p0 = axis_pass_h16( KIND_R2C, false, 2, 3, L.n, L.sr, L.sc, BUF_IN, BUF_WORK, n2, nullptr, yP0, rc );
+    p0.yoff    = E0;
+    if ( rc )
+        return rc;
+    pass_rt p1 = axis_pass_h16( KIND_C2C, false, 1, 3, extc, L.sc, L.sc, BUF_WORK, BUF_WORK, n1, yP0, yP1, rc );
+    p1.xoff    = E0;
+    p1.yoff    = E1;
+    if ( rc )
+        return rc;
+    pass_rt p2 = axis_pass_h16( KIND_C2C, false, 0, 3, extc, L.sc, L.sc, BUF_WORK, BUF_OUT, n0, yP1, nullptr, rc );
+    p2.xoff    = E1;
+    if ( rc )
+        return rc;
+    pass_rt q0 = axis_pass_h16( KIND_C2C, true, 0, 3, extc, L.sc, L.sc, BUF_IN, BUF_WORK, n0, nullptr, yQ0, rc );
+    q0.yoff    = E0;
+    if ( rc )
+        return rc;
+    pass_rt q1 = axis_pass_h16( KIND_C2C, true, 1, 3, extc, L.sc, L.sc, BUF_WORK, BUF_WORK, n1, yQ0, yQ1, rc );
+    q1.xoff    = E0;
+    q1.yoff    = E1;
+    if ( rc )
+        return rc;
+    pass_rt q2 = axis_pass_h16( KIND_C2R, true, 2, 3, L.n, L.sc, L.sr, BUF_WORK, BUF_OUT, n2, yQ1, nullptr, rc );
+    q2.xoff    = E1;
+    if ( rc )
+        return rc;
+    fwd = { p0, p1, p2 };
+    bwd = { q0, q1, q2 };
+    return ok();
+}
+
 } // namespace
 } // namespace fftm_mi
 
@@ -782,7 +949,9 @@ struct fftm_mi_plan_s
 struct fftm_mi_ffts_s
 {
     int32_t              layout = 0;
-    int32_t              prec   = 0;
+    int32_t              prec   = 0;     // arithmetic precision (FP32 for FFTM_MI_F32_H16)
+    bool                 h16    = false; // FP16-stored intermediate spectra
+    bool                 fwd_work = false; // forward passes use the work area
     int32_t              flags  = 0;
     int64_t              n[4]   = { 0, 0, 0, 0 };
     std::vector<pass_rt> fwd, bwd;
@@ -792,6 +961,10 @@ struct fftm_mi_ffts_s
     hipStream_t          stream    = nullptr;
     spec_geom            sg{};
     double              *partials  = nullptr; // shell-gradient partial sums (SHELL_BLOCKS x MAX_SHELLS)
+    // filter chain real -> real: forward passes into the work area, the fused filter pass
+    // (or the forward passes, an elementwise multiply and the backward passes)
+    std::vector<pass_rt> flt_pre, flt_post;
+    bool                 flt_ready = false, flt_fused = false;
 
     ~fftm_mi_ffts_s()
     {
@@ -1041,7 +1214,7 @@ int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, 
     g_err.clear();
     if ( !n )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_validate: null sizes" );
-    if ( precision != FFTM_MI_F32 && precision != FFTM_MI_F64 )
+    if ( precision != FFTM_MI_F32 && precision != FFTM_MI_F64 && precision != FFTM_MI_F32_H16 )
         return fail( FFTM_MI_INVALID_TYPE, "fftm_mi_ffts_validate: unknown precision" );
     const int dim = layout == FFTM_MI_LAYOUT_3D ? 3 : 4;
     for ( int k = 0; k < dim; ++k )
@@ -1051,6 +1224,19 @@ int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, 
     int         rc = make_ffts_layout( layout, n, L );
     if ( rc != FFTM_MI_SUCCESS )
         return rc;
+    if ( precision == FFTM_MI_F32_H16 )
+    {
+        std::vector<pass_rt> f, b;
+        size_t               wb = 0;
+        rc                      = build_ffts_passes_h16( L, f, b, wb );
+        if ( rc == FFTM_MI_SUCCESS )
+            rc = prepare_passes( f, FFTM_MI_F32, nullptr );
+        if ( rc == FFTM_MI_SUCCESS )
+            rc = prepare_passes( b, FFTM_MI_F32, nullptr );
+        if ( rc == FFTM_MI_SUCCESS && work_size )
+            *work_size = wb;
+        return rc;
+    }
     std::vector<pass_rt> f, b;
     int64_t              we = 0;
     rc = build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, f, b, we );
@@ -1081,21 +1267,31 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
     if ( !p )
         return fail( FFTM_MI_ALLOC_FAILED, "fftm_mi_ffts_create: out of host memory" );
     p->layout  = layout;
-    p->prec    = precision;
+    p->h16     = precision == FFTM_MI_F32_H16;
+    p->prec    = p->h16 ? FFTM_MI_F32 : precision;
     p->flags   = flags;
-    p->tb.prec = precision;
+    p->tb.prec = p->prec;
     const int dim = layout == FFTM_MI_LAYOUT_3D ? 3 : 4;
     for ( int k = 0; k < dim; ++k )
         p->n[k] = n[k];
     ffts_layout L;
     make_ffts_layout( layout, n, L );
-    int64_t we = 0;
-    build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, p->fwd, p->bwd, we );
-    if ( !( flags & FFTM_MI_FLAG_NO_PLANE_FUSION ) )
-        fuse_plane( L, layout, precision, p->fwd, p->bwd );
-    rc = prepare_passes( p->fwd, precision, &p->tb );
+    if ( p->h16 )
+    {
+        size_t wb = 0;
+        build_ffts_passes_h16( L, p->fwd, p->bwd, wb );
+        p->fwd_work = true;
+    }
+    else
+    {
+        int64_t we = 0;
+        build_ffts_passes( L, ( flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) != 0, p->fwd, p->bwd, we );
+        if ( !( flags & FFTM_MI_FLAG_NO_PLANE_FUSION ) )
+            fuse_plane( L, layout, precision, p->fwd, p->bwd );
+    }
+    rc = prepare_passes( p->fwd, p->prec, &p->tb );
     if ( rc == FFTM_MI_SUCCESS )
-        rc = prepare_passes( p->bwd, precision, &p->tb );
+        rc = prepare_passes( p->bwd, p->prec, &p->tb );
     if ( rc != FFTM_MI_SUCCESS )
         return rc;
     p->work_size = ws;
@@ -1143,6 +1339,8 @@ int fftm_mi_ffts_forward( fftm_mi_ffts h, const void *in, void *out, int32_t syn
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_forward: null data pointer" );
     if ( in == out )
         return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_forward: in-place transform is not supported" );
+    if ( h->fwd_work && !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_forward: a work area is required (set_work_area)" );
     return run_passes( h->fwd, h->tb, h->prec, in, out, h->work, h->stream, false, sync != 0, "ffts_forward" );
 }
 
@@ -1179,7 +1377,7 @@ int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, cons
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_run_pass: pass index out of range" );
     if ( !in || !out )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_run_pass: null data pointer" );
-    if ( backward && h->work_size > 0 && !h->work )
+    if ( ( backward || h->fwd_work ) && h->work_size > 0 && !h->work )
         return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_run_pass: a work area is required (set_work_area)" );
     std::vector<pass_rt> one( 1, v[(size_t)index] );
     return run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, sync != 0, "ffts_run_pass" );
@@ -1195,6 +1393,20 @@ int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, in
     const pass_rt &p  = v[(size_t)index];
     const int64_t  cs = csize_of( h->prec ), rs = cs / 2;
     const int64_t  nh = p.N / 2 + 1;
+    if ( p.iom )
+    {
+        // FP16 side: 4 bytes per bin; exponents: one byte per stored line
+        const int64_t lines = p.kind == KIND_C2C ? p.ncols : p.ncols;
+        const int64_t ic = ( p.iom & IO_H16_IN ) ? 4 : cs, oc = ( p.iom & IO_H16_OUT ) ? 4 : cs;
+        const int64_t ex = ( ( p.iom & IO_H16_IN ) ? lines : 0 ) + ( ( p.iom & IO_H16_OUT ) ? lines : 0 );
+        if ( p.kind == KIND_C2C )
+            *bytes = p.ncols * p.N * ( ic + oc ) + ex;
+        else if ( p.kind == KIND_R2C )
+            *bytes = p.ncols * ( p.N * rs + nh * oc ) + ex;
+        else
+            *bytes = p.ncols * ( nh * ic + p.N * rs ) + ex;
+        return ok();
+    }
     if ( p.plane )
         *bytes = p.nplanes * p.NB * ( p.N * rs + nh * cs );
     else if ( p.kind == KIND_C2C )
@@ -1308,6 +1520,157 @@ int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, cons
     return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_shell_gradient" );
 }
 
+} // extern "C"
+
+namespace
+{
+
+int check_op( const fftm_mi_ffts_s &h, const fftm_mi_spectral_op *op, const char *what )
+{
+    if ( !op )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": null operator" );
+    if ( op->kind < FFTM_MI_OP_POISSON || op->kind > FFTM_MI_OP_SHELL )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": unknown operator kind" );
+    if ( op->kind == FFTM_MI_OP_GAUSSIAN && !( op->sigma > 0.0 ) )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": Gaussian sigma must be > 0" );
+    if ( op->kind == FFTM_MI_OP_SHELL && ( !op->coeffs || op->nshells < 1 || op->nshells > MAX_SHELLS ) )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": shell operator needs coeffs and 1..1024 shells" );
+    (void)h;
+    return ok();
+}
+
+template <typename T>
+filter_params<T> make_filter_params( const fftm_mi_ffts_s &h, const fftm_mi_spectral_op &op )
+{
+    filter_params<T> f{};
+    f.kind    = op.kind;
+    f.nshells = op.kind == FFTM_MI_OP_SHELL ? op.nshells : 1;
+    f.nd      = h.sg.nd;
+    for ( int m = 0; m < h.sg.nd; ++m )
+    {
+        const int a = h.sg.ax[m];
+        f.ext[m]    = h.sg.ext[m];
+        f.nlen[m]   = a == h.sg.half_axis ? 0 : h.sg.n[a];
+    }
+    f.scale          = op.scale;
+    f.inv_two_sigma2 = op.kind == FFTM_MI_OP_GAUSSIAN ? 1.0 / ( 2.0 * op.sigma * op.sigma ) : 0.0;
+    f.kscale         = h.sg.kscale * (double)f.nshells;
+    f.coeffs         = static_cast<const T *>( op.coeffs );
+    return f;
+}
+
+bool same_lines( const pass_rt &a, const pass_rt &b )
+{
+    return a.N == b.N && a.ies == b.ies && a.oes == b.oes && a.ncols == b.ncols && a.d0 == b.d0 && a.d1 == b.d1 &&
+           a.is0 == b.is0 && a.is1 == b.is1 && a.is2 == b.is2 && a.os0 == b.os0 && a.os1 == b.os1 && a.os2 == b.os2;
+}
+
+// The real -> real filter chain: forward passes with the spectrum in the work area, then
+// either the fused filter pass (last forward axis == first backward axis) or an
+// elementwise multiply, then the remaining backward passes.
+int ensure_filter_chain( fftm_mi_ffts_s &h )
+{
+    if ( h.flt_ready )
+        return ok();
+    if ( h.h16 )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_filter: not implemented for FP16-stored spectra" );
+    if ( h.flags & FFTM_MI_FLAG_OVERWRITE_INPUT )
+        return fail( FFTM_MI_NOT_SUPPORTED,
+                     "fftm_mi_ffts_filter: keeps the spectrum in the work area (create without OVERWRITE_INPUT)" );
+    std::vector<pass_rt> pre = h.fwd, post = h.bwd;
+    for ( pass_rt &r : pre )
+    {
+        if ( r.src == BUF_OUT )
+            r.src = BUF_WORK;
+        if ( r.dst == BUF_OUT )
+            r.dst = BUF_WORK;
+    }
+    for ( pass_rt &r : post )
+        if ( r.src == BUF_IN )
+            r.src = BUF_WORK;
+    const pass_rt &lf = pre.back(), &fb = post.front();
+    const bool     fusable = !lf.plane && !lf.generic && lf.kind == KIND_C2C && !lf.inv && !fb.plane && !fb.generic &&
+                         fb.kind == KIND_C2C && fb.inv && same_lines( lf, fb ) && fixed_supported( lf.N ) &&
+                         lf.src == BUF_WORK && lf.dst == BUF_WORK && fb.dst == BUF_WORK;
+    if ( fusable )
+    {
+        pre.back().filter = true;
+        pre.back().var    = 0;
+        post.erase( post.begin() );
+    }
+    h.flt_pre   = pre;
+    h.flt_post  = post;
+    h.flt_fused = fusable;
+    h.flt_ready = true;
+    return ok();
+}
+
+template <typename T>
+int spectral_op_launch( fftm_mi_ffts_s &h, const filter_params<T> &f, const void *in, void *out )
+{
+    const int64_t total = h.sg.total;
+    const int64_t blocks = std::min<int64_t>( ( total + 255 ) / 256, 8192 );
+    hipLaunchKernelGGL( k_spectral_op<T>, dim3( (unsigned)std::max<int64_t>( blocks, 1 ) ), dim3( 256 ), 0, h.stream,
+                        f, static_cast<const cpx<T> *>( in ), static_cast<cpx<T> *>( out ), total );
+    const hipError_t e = hipGetLastError();
+    if ( e != hipSuccess )
+        return fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi: spectral operator launch failed: " ) + hipGetErrorString( e ) );
+    return ok();
+}
+
+template <typename T>
+int filter_run( fftm_mi_ffts_s &h, const void *in, void *out, const fftm_mi_spectral_op &op, bool sync )
+{
+    const filter_params<T> f  = make_filter_params<T>( h, op );
+    int                    rc = run_passes( h.flt_pre, h.tb, h.prec, in, out, h.work, h.stream, false, false,
+                                            "ffts_filter", 0, &f );
+    if ( rc == FFTM_MI_SUCCESS && !h.flt_fused )
+        rc = spectral_op_launch<T>( h, f, h.work, h.work );
+    if ( rc == FFTM_MI_SUCCESS )
+        rc = run_passes( h.flt_post, h.tb, h.prec, in, out, h.work, h.stream, false, sync, "ffts_filter" );
+    return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int fftm_mi_ffts_spectral_apply( fftm_mi_ffts h, const void *spectrum_in, void *spectrum_out,
+                                 const fftm_mi_spectral_op *op, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_spectral_apply: null handle" );
+    if ( !spectrum_in || !spectrum_out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_spectral_apply: null data pointer" );
+    int rc = check_op( *h, op, "fftm_mi_ffts_spectral_apply" );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    rc = h->prec == FFTM_MI_F32 ? spectral_op_launch<float>( *h, make_filter_params<float>( *h, *op ), spectrum_in, spectrum_out )
+                                : spectral_op_launch<double>( *h, make_filter_params<double>( *h, *op ), spectrum_in, spectrum_out );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_spectral_apply" );
+}
+
+int fftm_mi_ffts_filter( fftm_mi_ffts h, const void *real_in, void *real_out, const fftm_mi_spectral_op *op,
+                         int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_filter: null handle" );
+    if ( !real_in || !real_out )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_filter: null data pointer" );
+    int rc = check_op( *h, op, "fftm_mi_ffts_filter" );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    rc = ensure_filter_chain( *h );
+    if ( rc != FFTM_MI_SUCCESS )
+        return rc;
+    if ( !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_filter: a work area is required (set_work_area)" );
+    return h->prec == FFTM_MI_F32 ? filter_run<float>( *h, real_in, real_out, *op, sync != 0 )
+                                  : filter_run<double>( *h, real_in, real_out, *op, sync != 0 );
+}
+
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant )
 {
     if ( !h )
@@ -1317,7 +1680,10 @@ int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t ind
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: pass index out of range" );
     if ( variant < 0 || variant >= NUM_VARIANTS )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: variant out of range" );
+    if ( v[(size_t)index].iom && !mixed_variant_ok( variant ) )
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_set_pass_variant: FP16-storage passes have variants 0, 6, 7" );
     v[(size_t)index].var = variant;
+    h->flt_ready         = false;
     return ok();
 }
 
@@ -1451,7 +1817,7 @@ std::string problem_key( const fftm_mi_ffts_s &h )
     o << "FFTM_MI_TILE_" << names[h.layout] << "_" << h.n[0] << "x" << h.n[1] << "x" << h.n[2];
     if ( h.layout != FFTM_MI_LAYOUT_3D )
         o << "x" << h.n[3];
-    o << ( h.prec == FFTM_MI_F32 ? "_F32" : "_F64" );
+    o << ( h.h16 ? "_F32_H16" : ( h.prec == FFTM_MI_F32 ? "_F32" : "_F64" ) );
     return o.str();
 }
 
@@ -1588,7 +1954,8 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             // the pass writes into: out (its dst) -- compare the whole destination buffer
             const pass_rt &cur     = v[i];
             void          *dstbuf  = cur.dst == BUF_OUT ? out : ( cur.dst == BUF_WORK ? h->work : const_cast<void *>( in ) );
-            const size_t   dbytes  = cur.dst == BUF_OUT ? ( bw ? rbytes : cbytes ) : cbytes;
+            const size_t   dbytes  = cur.dst == BUF_OUT ? ( bw ? rbytes : cbytes )
+                                                        : ( cur.dst == BUF_WORK ? h->work_size : cbytes );
             std::vector<pass_rt> prefix( v.begin(), v.begin() + (long)i );
             std::vector<pass_rt> one( 1, cur );
             double               best_ms = 1e300;
@@ -1596,6 +1963,8 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             const int            nvar    = ( cur.plane || cur.generic ) ? 1 : NUM_VARIANTS;
             for ( int var = 0; var < nvar && rc == 0; ++var )
             {
+                if ( cur.iom && !mixed_variant_ok( var ) )
+                    continue;
                 one[0].var = var;
                 // regenerate the pass input, run once, check against variant 0
                 if ( bw )
@@ -1652,6 +2021,7 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             }
             if ( rc == 0 )
             {
+                h->flt_ready                    = false;
                 v[i].var                        = best;
                 rec[pass_key( prob, bw, i )]    = std::to_string( best );
                 std::ostringstream mo;

@@ -58,6 +58,24 @@ int fixed_tile_cols_f64( int N, bool loadb, int var )
     }
 }
 
+#define FFTM_MI_MIXED_LENGTHS( X ) X( 128 ) X( 256 ) X( 512 ) X( 1024 ) X( 2048 )
+#define FFTM_MI_DECLARE_MIXED( N )                                                                                     \
+    hipError_t launch_mixed_f32_n##N( const pass_params<float> &, int, bool, bool, int, int, hipStream_t );
+#define FFTM_MI_CASE_MIXED( N )                                                                                        \
+    case N: return launch_mixed_f32_n##N( p, kind, inv, loadb, var, iom, s );
+
+FFTM_MI_MIXED_LENGTHS( FFTM_MI_DECLARE_MIXED )
+
+hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, int iom,
+                                  hipStream_t s )
+{
+    switch ( N )
+    {
+        FFTM_MI_MIXED_LENGTHS( FFTM_MI_CASE_MIXED )
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
                                hipStream_t s )
 {

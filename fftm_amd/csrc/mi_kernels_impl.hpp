@@ -16,17 +16,17 @@ constexpr int variants_for()
     return N >= 64 ? NUM_VARIANTS : 1;
 }
 
-template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR>
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32>
 hipError_t launch_fixed( const pass_params<T> &p, hipStream_t s )
 {
-    using K              = pass_kernel<T, N, KIND, INV, LOADB, VAR>;
+    using K              = pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>;
     const int64_t per    = (int64_t)K::TW * K::CPT;
     const int64_t ntiles = ( p.ncols + per - 1 ) / per;
     if ( ntiles <= 0 )
         return hipSuccess;
     if ( ntiles > 0x7fffffffLL )
         return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL( ( k_pass<T, N, KIND, INV, LOADB, VAR> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p );
+    hipLaunchKernelGGL( ( k_pass<T, N, KIND, INV, LOADB, VAR, IOM> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p );
     return hipGetLastError();
 }
 
@@ -60,6 +60,8 @@ hipError_t launch_n( const pass_params<T> &p, int kind, bool inv, bool loadb, in
         case 3: return launch_nv<T, N, 3>( p, kind, inv, loadb, s );
         case 4: return launch_nv<T, N, 4>( p, kind, inv, loadb, s );
         case 5: return launch_nv<T, N, 5>( p, kind, inv, loadb, s );
+        case 6: return launch_nv<T, N, 6>( p, kind, inv, loadb, s );
+        case 7: return launch_nv<T, N, 7>( p, kind, inv, loadb, s );
         default: break;
         }
     }
@@ -84,10 +86,84 @@ int tile_cols_n( bool loadb, int var )
         case 3: return tile_cols_nv<T, N, 3>( loadb );
         case 4: return tile_cols_nv<T, N, 4>( loadb );
         case 5: return tile_cols_nv<T, N, 5>( loadb );
+        case 6: return tile_cols_nv<T, N, 6>( loadb );
+        case 7: return tile_cols_nv<T, N, 7>( loadb );
         default: break;
         }
     }
     return tile_cols_nv<T, N, 0>( loadb );
+}
+
+// fused filter pass (forward C2C, x H(k), inverse C2C), default tile shape
+template <typename T, int N>
+hipError_t launch_filter_n( const pass_params<T> &p, const filter_params<T> &f, bool loadb, hipStream_t s )
+{
+    auto go = [&]( auto lb ) -> hipError_t {
+        constexpr bool LB    = decltype( lb )::value;
+        using K              = pass_kernel<T, N, KIND_C2C, false, LB, 0, IO_F32, true>;
+        const int64_t ntiles = ( p.ncols + K::TW - 1 ) / K::TW;
+        if ( ntiles <= 0 )
+            return hipSuccess;
+        if ( ntiles > 0x7fffffffLL )
+            return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL( ( k_pass_filter<T, N, LB, 0> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p, f );
+        return hipGetLastError();
+    };
+    return loadb ? go( std::true_type{} ) : go( std::false_type{} );
+}
+
+// FP16-stored spectra (FP32 arithmetic): the storage modes each kind can have
+template <int N, int VAR, int IOM>
+hipError_t launch_mixed_io( const pass_params<float> &p, int kind, bool inv, bool loadb, hipStream_t s )
+{
+    if ( kind == KIND_C2C )
+    {
+        if ( inv )
+            return loadb ? launch_fixed<float, N, KIND_C2C, true, true, VAR, IOM>( p, s )
+                         : launch_fixed<float, N, KIND_C2C, true, false, VAR, IOM>( p, s );
+        return loadb ? launch_fixed<float, N, KIND_C2C, false, true, VAR, IOM>( p, s )
+                     : launch_fixed<float, N, KIND_C2C, false, false, VAR, IOM>( p, s );
+    }
+    if constexpr ( IOM == IO_H16_OUT )
+    {
+        if ( kind == KIND_R2C )
+            return loadb ? launch_fixed<float, N, KIND_R2C, false, true, VAR, IOM>( p, s )
+                         : launch_fixed<float, N, KIND_R2C, false, false, VAR, IOM>( p, s );
+    }
+    if constexpr ( IOM == IO_H16_IN )
+    {
+        if ( kind == KIND_C2R )
+            return loadb ? launch_fixed<float, N, KIND_C2R, true, true, VAR, IOM>( p, s )
+                         : launch_fixed<float, N, KIND_C2R, true, false, VAR, IOM>( p, s );
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int N, int VAR>
+hipError_t launch_mixed_v( const pass_params<float> &p, int kind, bool inv, bool loadb, int iom, hipStream_t s )
+{
+    switch ( iom )
+    {
+    case IO_H16_IN: return launch_mixed_io<N, VAR, IO_H16_IN>( p, kind, inv, loadb, s );
+    case IO_H16_OUT: return launch_mixed_io<N, VAR, IO_H16_OUT>( p, kind, inv, loadb, s );
+    case IO_H16_IN | IO_H16_OUT: return launch_mixed_io<N, VAR, IO_H16_IN | IO_H16_OUT>( p, kind, inv, loadb, s );
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// tile-shape variants compiled for the FP16-storage kernels
+constexpr bool mixed_variant( int var ) { return var == 0 || var == 6 || var == 7; }
+
+template <int N>
+hipError_t launch_mixed_n( const pass_params<float> &p, int kind, bool inv, bool loadb, int var, int iom, hipStream_t s )
+{
+    switch ( var )
+    {
+    case 0: return launch_mixed_v<N, 0>( p, kind, inv, loadb, iom, s );
+    case 6: return launch_mixed_v<N, 6>( p, kind, inv, loadb, iom, s );
+    case 7: return launch_mixed_v<N, 7>( p, kind, inv, loadb, iom, s );
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>
@@ -131,5 +207,42 @@ hipError_t launch_generic_t( const pass_params<T> &p, const radix_list &rl, int 
     int tile_cols_##TN##_n##N( bool loadb, int var )                                                                   \
     {                                                                                                                  \
         return detail::tile_cols_n<T, N>( loadb, var );                                                                \
+    }                                                                                                                  \
+    }
+
+// One length N of the FP16-storage kernels: defines launch_mixed_f32_n<N>.
+#define FFTM_MI_DEFINE_MIXED_LENGTH( N )                                                                               \
+    namespace fftm_mi                                                                                                  \
+    {                                                                                                                  \
+    hipError_t launch_mixed_f32_n##N( const pass_params<float> &p, int kind, bool inv, bool loadb, int var, int iom,   \
+                                      hipStream_t s )                                                                  \
+    {                                                                                                                  \
+        return detail::launch_mixed_n<N>( p, kind, inv, loadb, var, iom, s );                                          \
+    }                                                                                                                  \
+    }
+
+// The fused filter pass for precision T (type name TN) over every compiled length.
+#define FFTM_MI_DEFINE_FILTER( T, TN )                                                                                 \
+    namespace fftm_mi                                                                                                  \
+    {                                                                                                                  \
+    hipError_t launch_filter_##TN( const pass_params<T> &p, const filter_params<T> &f, int N, bool loadb,              \
+                                   hipStream_t s )                                                                     \
+    {                                                                                                                  \
+        switch ( N )                                                                                                   \
+        {                                                                                                              \
+        case 2: return detail::launch_filter_n<T, 2>( p, f, loadb, s );                                                \
+        case 4: return detail::launch_filter_n<T, 4>( p, f, loadb, s );                                                \
+        case 8: return detail::launch_filter_n<T, 8>( p, f, loadb, s );                                                \
+        case 16: return detail::launch_filter_n<T, 16>( p, f, loadb, s );                                              \
+        case 32: return detail::launch_filter_n<T, 32>( p, f, loadb, s );                                              \
+        case 64: return detail::launch_filter_n<T, 64>( p, f, loadb, s );                                              \
+        case 128: return detail::launch_filter_n<T, 128>( p, f, loadb, s );                                            \
+        case 256: return detail::launch_filter_n<T, 256>( p, f, loadb, s );                                            \
+        case 512: return detail::launch_filter_n<T, 512>( p, f, loadb, s );                                            \
+        case 1024: return detail::launch_filter_n<T, 1024>( p, f, loadb, s );                                          \
+        case 2048: return detail::launch_filter_n<T, 2048>( p, f, loadb, s );                                          \
+        case 4096: return detail::launch_filter_n<T, 4096>( p, f, loadb, s );                                          \
+        default: return hipErrorInvalidValue;                                                                          \
+        }                                                                                                              \
     }                                                                                                                  \
     }

@@ -1,0 +1,6 @@
+// FP16-storage pass kernels (FP32 arithmetic), lengths 128, 256, 2048.
+#include "mi_kernels_impl.hpp"
+
+FFTM_MI_DEFINE_MIXED_LENGTH( 128 )
+FFTM_MI_DEFINE_MIXED_LENGTH( 256 )
+FFTM_MI_DEFINE_MIXED_LENGTH( 2048 )

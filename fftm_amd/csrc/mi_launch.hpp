@@ -28,6 +28,25 @@ bool       plane_supported( int64_t na, int64_t nb, int prec );
 hipError_t launch_plane_f32( const plane_params<float> &p, int na, int nb, bool fwd, hipStream_t s );
 hipError_t launch_plane_f64( const plane_params<double> &p, int na, int nb, bool fwd, hipStream_t s );
 
+// FP16-stored spectra with FP32 arithmetic (iom: bit 0 FP16 input, bit 1 FP16 output);
+// lengths 128..2048, tile-shape variants 0, 6, 7
+inline bool mixed_supported( int64_t N )
+{
+    return N >= 128 && N <= 2048 && ( N & ( N - 1 ) ) == 0;
+}
+inline bool mixed_variant_ok( int var )
+{
+    return var == 0 || var == 6 || var == 7;
+}
+hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, int iom,
+                                  hipStream_t s );
+
+// fused filter pass along a compiled length (forward C2C, x H(k), inverse C2C)
+hipError_t launch_filter_f32( const pass_params<float> &p, const filter_params<float> &f, int N, bool loadb,
+                              hipStream_t s );
+hipError_t launch_filter_f64( const pass_params<double> &p, const filter_params<double> &f, int N, bool loadb,
+                              hipStream_t s );
+
 hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
                                hipStream_t s );
 hipError_t launch_generic_f64( const pass_params<double> &p, const radix_list &rl, int N, int cols, int kind, bool inv,

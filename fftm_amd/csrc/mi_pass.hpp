@@ -30,6 +30,8 @@
 
 #include "mi_complex.hpp"
 
+#include <type_traits>
+
 namespace fftm_mi
 {
 
@@ -53,7 +55,114 @@ struct pass_params
     int64_t       os0, os1, os2;
     int32_t       store_b; // 1: store with the column-fastest mapping, 0: line-fastest
     int32_t       wmode;   // real-axis bin weights: 0 none, 1 R2C output x w_k, 2 C2R input / w_k
+    int32_t       vec;     // bit 0: paired (8/16-byte) loads allowed, bit 1: paired stores allowed
     T             scale;   // multiplies every output (1 for the plain transforms)
+    // FP16-stored spectra (mixed precision): every stored line carries a power-of-two
+    // exponent; element value = FP16 value * 2^exponent.  Exponent indices are linear in
+    // the coordinates, like the data offsets.
+    const int8_t *xin;                  // exponents of the FP16 input lines
+    int8_t       *xout;                 // exponents of the FP16 output lines (written here)
+    int64_t       xs0, xs1, xs2, xsn;   // input exponent strides: batch dims, transform axis
+    int64_t       ys0, ys1, ys2;        // output exponent strides: batch dims
+};
+
+typedef _Float16 h16x2 __attribute__( ( ext_vector_type( 2 ) ) );
+
+// storage modes of a pass (template parameter IOM): bit 0 FP16 input, bit 1 FP16 output
+enum : int
+{
+    IO_F32    = 0,
+    IO_H16_IN = 1,
+    IO_H16_OUT = 2
+};
+
+// two consecutive elements of T (8 bytes for float, 16 for double) and two consecutive
+// FP32 complex values (16 bytes)
+template <typename T>
+struct pair_of;
+template <>
+struct pair_of<float>
+{
+    typedef float type __attribute__( ( ext_vector_type( 2 ) ) );
+};
+template <>
+struct pair_of<double>
+{
+    typedef double type __attribute__( ( ext_vector_type( 2 ) ) );
+};
+typedef float f32x4 __attribute__( ( ext_vector_type( 4 ) ) );
+
+// Diagonal spectral operator applied between a forward and an inverse transform of the
+// same lines (the fused filter pass): H(k) from the wavenumber vector of each bin.
+enum : int
+{
+    OP_POISSON  = 1, // H = -scale / |k|^2, H(0) = 0   (periodic -Laplace inverse)
+    OP_GAUSSIAN = 2, // H = scale * exp(-|k|^2 / (2 sigma^2))
+    OP_SHELL    = 3  // H = scale * coeffs[min(S - 1, floor(S |k| / kmax))]
+};
+
+template <typename T>
+struct filter_params
+{
+    int32_t  kind;
+    int32_t  nshells;
+    int32_t  nd;       // memory axes of the (dense) spectrum array, slowest first
+    int32_t  pad_;
+    int64_t  ext[4];   // memory extents
+    int64_t  nlen[4];  // full logical length of each memory axis; 0 marks the halved real axis
+    double   scale;
+    double   inv_two_sigma2;
+    double   kscale;   // S / kmax
+    const T *coeffs;   // OP_SHELL coefficients (T[nshells])
+};
+
+// sum of k_a^2 over the memory axes, for the bin at spectrum offset off
+// (wavenumbers: k = i on the halved axis, k = i <= n/2 ? i : i - n elsewhere)
+template <typename T>
+__device__ __forceinline__ float filter_k2( const filter_params<T> &f, int64_t off )
+{
+    float k2 = 0.f;
+    for ( int m = f.nd - 1; m >= 0; --m )
+    {
+        const int64_t idx = off % f.ext[m];
+        off /= f.ext[m];
+        const int64_t n = f.nlen[m];
+        const int64_t k = ( n == 0 || idx <= n / 2 ) ? idx : idx - n;
+        k2 += (float)( k * k );
+    }
+    return k2;
+}
+
+template <typename T>
+__device__ __forceinline__ T filter_h( const filter_params<T> &f, float k2 )
+{
+    if ( f.kind == OP_POISSON )
+        return k2 == 0.f ? (T)0 : (T)( -f.scale ) / (T)k2;
+    if ( f.kind == OP_GAUSSIAN )
+    {
+        if constexpr ( sizeof( T ) == 4 )
+            return (T)f.scale * __expf( -k2 * (float)f.inv_two_sigma2 );
+        else
+            return (T)( f.scale * exp( -(double)k2 * f.inv_two_sigma2 ) );
+    }
+    int s = (int)( sqrtf( k2 ) * (float)f.kscale );
+    s     = s < f.nshells - 1 ? s : f.nshells - 1;
+    return (T)f.scale * f.coeffs[s];
+}
+
+// per-workgroup state of a pass kernel (shared by the forward and inverse halves of
+// the fused filter pass)
+template <typename T, typename LW>
+struct pass_ctx
+{
+    const pass_params<T>   *p;
+    const filter_params<T> *f;
+    LW                     *lds;
+    int                     c, t;
+    int64_t                 tile;
+    int64_t                 ia, ib, oa, ob;
+    int64_t                 xa, xb, ya, yb; // exponent indices (FP16 storage)
+    bool                    va, vb;
 };
 
 // Runtime description for the generic (any-radix) kernel.
@@ -147,7 +256,13 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 //   3: both (same workgroup size, half the tile)
 //   4: twice the columns (256-byte row segments; LDS up to 160 KB on gfx950)
 //   5: twice the columns and twice the threads per column (1024-thread workgroups)
-constexpr int NUM_VARIANTS = 6;
+//   6: the tile of 0, exchanged through LDS in two rounds (real parts, then imaginary
+//      parts): half the LDS per workgroup, so more tiles in flight per CU
+//   7: the tile of 4, exchanged in two rounds
+constexpr int NUM_VARIANTS = 8;
+
+constexpr bool variant_split( int var ) { return var >= 6; }
+constexpr int  variant_shape( int var ) { return var == 6 ? 0 : ( var == 7 ? 4 : var ); }
 
 constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 {
@@ -179,6 +294,10 @@ constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
         int tpc = hi > 256 ? 256 : hi;
         if ( tpc < lo )
             tpc = lo;
+        // two butterflies per thread when the tile still fits: the first and last stages
+        // then move pairs of consecutive elements (8- or 16-byte accesses)
+        if ( tpc >= 2 && tpc / 2 >= lo && ( 256 / ( tpc / 2 ) ) * geom_pa( N, 3 ) * csize <= lds_limit_bytes() )
+            tpc /= 2;
         int tw  = 256 / tpc > 0 ? 256 / tpc : 1;
         int pad = 3;
         while ( tw > 1 && tw * geom_pa( N, pad ) * csize > lds_limit_bytes() )
@@ -219,6 +338,78 @@ __device__ __forceinline__ void col_offsets( const pass_params<T> &p, int64_t co
     const int64_t c2 = r / p.d1;
     io               = c0 * p.is0 + c1 * p.is1 + c2 * p.is2;
     oo               = c0 * p.os0 + c1 * p.os1 + c2 * p.os2;
+}
+
+// exponent indices of column col: input lines (consumer side), output line (producer side)
+template <typename T>
+__device__ __forceinline__ void col_exponents( const pass_params<T> &p, int64_t col, int64_t &xi, int64_t &yo )
+{
+    const int64_t c0 = col % p.d0;
+    const int64_t r  = col / p.d0;
+    const int64_t c1 = r % p.d1;
+    const int64_t c2 = r / p.d1;
+    xi               = c0 * p.xs0 + c1 * p.xs1 + c2 * p.xs2;
+    yo               = c0 * p.ys0 + c1 * p.ys1 + c2 * p.ys2;
+}
+
+// FP16 element * 2^e in FP32
+__device__ __forceinline__ cpx<float> h16_load( const void *base, int64_t idx, const int8_t *xp, int64_t xidx )
+{
+    const h16x2 h = static_cast<const h16x2 *>( base )[idx];
+    const int   e = xp[xidx];
+    return mk<float>( __builtin_ldexpf( (float)h.x, e ), __builtin_ldexpf( (float)h.y, e ) );
+}
+
+// exponent of a line whose largest component magnitude is amax: the stored values
+// amax * 2^-e then lie in [2^14, 2^15), inside the FP16 range (65504)
+__device__ __forceinline__ int h16_exponent( float amax )
+{
+    int e = 0;
+    (void)__builtin_frexpf( amax, &e );
+    e -= 15;
+    return e < -127 ? -127 : ( e > 127 ? 127 : e );
+}
+
+__device__ __forceinline__ h16x2 h16_pack( float a, float b, int e )
+{
+    return h16x2{ (_Float16)__builtin_ldexpf( a, -e ), (_Float16)__builtin_ldexpf( b, -e ) };
+}
+
+// element n of an FP16-stored input line (C2C) or of the Hermitian extension of two
+// FP16-stored half spectra (C2R), as in load_line
+template <int KIND>
+__device__ __forceinline__ cpx<float> load_line_h16( const pass_params<float> &p, int64_t ia, int64_t ib, int64_t xa,
+                                                     int64_t xb, bool va, bool vb, int n, int N )
+{
+    if constexpr ( KIND == KIND_C2C )
+    {
+        if ( !va )
+            return mk<float>( 0, 0 );
+        return h16_load( p.in, ia + (int64_t)n * p.ies, p.xin, xa + (int64_t)n * p.xsn );
+    }
+    else
+    {
+        const int  h  = N / 2;
+        const int  nn = n <= h ? n : N - n;
+        cpx<float> a  = va ? h16_load( p.in, ia + (int64_t)nn * p.ies, p.xin, xa + (int64_t)nn * p.xsn ) : mk<float>( 0, 0 );
+        cpx<float> b  = vb ? h16_load( p.in, ib + (int64_t)nn * p.ies, p.xin, xb + (int64_t)nn * p.xsn ) : mk<float>( 0, 0 );
+        if ( n > h )
+        {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        if ( n == 0 || 2 * n == N )
+        {
+            a.y = 0;
+            b.y = 0;
+        }
+        else if ( p.wmode == 2 )
+        {
+            a = scl( a, 0.5f );
+            b = scl( b, 0.5f );
+        }
+        return mk<float>( a.x - b.y, a.y + b.x );
+    }
 }
 
 // element n of the (possibly paired) input line, as one complex value
@@ -290,12 +481,18 @@ __device__ __forceinline__ void store_line( const pass_params<T> &p, int64_t oa,
 // ---------------------------------------------------------------------------
 // the compile-time-N pass kernel
 // ---------------------------------------------------------------------------
-template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR = 0>
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR = 0, int IOM = IO_F32, bool FLT = false>
 struct pass_kernel
 {
+    static_assert( !FLT || ( KIND == KIND_C2C && !INV && IOM == IO_F32 && !variant_split( VAR ) ),
+                   "the fused filter pass is a forward C2C pass with whole-value LDS exchange" );
+    static_assert( IOM == IO_F32 || std::is_same<T, float>::value, "FP16 storage is for FP32 transforms" );
+    static_assert( !( IOM & IO_H16_IN ) || KIND != KIND_R2C, "R2C reads real FP32 input" );
+    static_assert( !( IOM & IO_H16_OUT ) || KIND != KIND_C2R, "C2R writes real FP32 output" );
     using C                    = cpx<T>;
     static constexpr rplan RP  = make_rplan( N );
-    static constexpr geom  G   = pick_geom( N, (int)sizeof( C ), LOADB, VAR );
+    static constexpr bool  SPLIT = variant_split( VAR );
+    static constexpr geom  G   = pick_geom( N, (int)sizeof( C ), LOADB, variant_shape( VAR ) );
     static constexpr int   TW  = G.tw;
     static constexpr int   TPC = G.tpc;
     static constexpr int   NT  = TW * TPC;
@@ -303,8 +500,10 @@ struct pass_kernel
     static constexpr int   PB  = TW;
     static constexpr int   LDS = LOADB ? N * PB : TW * PA;
     static constexpr int   CPT = KIND == KIND_C2C ? 1 : 2;
-    static_assert( LDS * (int)sizeof( C ) <= ( VAR >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
+    static_assert( LDS * (int)sizeof( C ) <= ( variant_shape( VAR ) >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
                    "LDS tile too large" );
+    // LDS words: complex values, or one component at a time with the split exchange
+    using LW = typename std::conditional<SPLIT, T, C>::type;
 
     __device__ static __forceinline__ int L( int n, int c )
     {
@@ -314,15 +513,125 @@ struct pass_kernel
             return c * PA + ( G.pad ? n + ( n >> G.pad ) : n );
     }
 
-    struct ctx
+    // Stage S handles butterflies j in [0, N / R_S).  Thread t takes j = t + TPC b, except
+    // in the first and last stages of the line-contiguous mapping, where it takes pairs
+    // (2 (t + TPC (b / 2)) + b % 2) so that its HBM accesses cover consecutive elements.
+    template <int S>
+    static constexpr bool pair_stage()
     {
-        const pass_params<T> *p;
-        C                    *lds;
-        int                   c, t;
-        int64_t               tile;
-        int64_t               ia, ib, oa, ob;
-        bool                  va, vb;
-    };
+        if constexpr ( LOADB )
+            return false;
+        else
+        {
+            constexpr int NBF = N / RP.r[S];
+            constexpr int NB  = ( NBF + TPC - 1 ) / TPC;
+            return ( S == 0 || S == RP.n - 1 ) && NB % 2 == 0 && NBF % 2 == 0;
+        }
+    }
+    template <int S>
+    __device__ static __forceinline__ int jmap( int t, int b )
+    {
+        if constexpr ( pair_stage<S>() )
+            return 2 * ( t + TPC * ( b >> 1 ) ) + ( b & 1 );
+        else
+            return t + TPC * b;
+    }
+    // paired HBM accesses exist for real lines (two reals) and FP32 complex lines
+    static constexpr bool PAIR_LOAD  = IOM == IO_F32 && pair_stage<0>() &&
+                                      ( KIND == KIND_R2C || ( KIND == KIND_C2C && sizeof( T ) == 4 ) );
+    static constexpr bool PAIR_STORE = IOM == IO_F32 && pair_stage<RP.n - 1>() &&
+                                       ( KIND == KIND_C2R || ( KIND == KIND_C2C && sizeof( T ) == 4 ) );
+
+    // largest component magnitude over the TPC threads that share a column: each thread
+    // passes its own maximum; column-fastest thread order when cf
+    __device__ static __forceinline__ float column_max( float *red, float lm, int c, bool cf )
+    {
+        red[threadIdx.x] = lm;
+        __syncthreads();
+        float m = 0.f;
+#pragma unroll 8
+        for ( int t = 0; t < TPC; ++t )
+            m = fmaxf( m, red[cf ? t * TW + c : c * TPC + t] );
+        return m;
+    }
+    __device__ static __forceinline__ float cmag( const C &z ) { return fmaxf( fabsf( (float)z.x ), fabsf( (float)z.y ) ); }
+
+    using ctx = pass_ctx<T, LW>;
+
+    // ---- LDS exchange helpers: P = 2 moves whole values, P = 0 / 1 the real /
+    // imaginary components (split exchange) ------------------------------------------
+    template <int P>
+    __device__ static __forceinline__ void lds_st( LW *l, int idx, const C &z )
+    {
+        if constexpr ( P == 2 )
+            l[idx] = z;
+        else if constexpr ( P == 0 )
+            l[idx] = z.x;
+        else
+            l[idx] = z.y;
+    }
+    template <int P>
+    __device__ static __forceinline__ void lds_ld( const LW *l, int idx, C &z )
+    {
+        if constexpr ( P == 2 )
+            z = l[idx];
+        else if constexpr ( P == 0 )
+            z.x = l[idx];
+        else
+            z.y = l[idx];
+    }
+    // results of stage S (radix R, span NS) into LDS at their Stockham positions
+    template <int P, int S, int R, int NS, int NB>
+    __device__ static __forceinline__ void put_stage( const C ( &v )[NB][R], ctx &x )
+    {
+        constexpr int NBF = N / R;
+#pragma unroll
+        for ( int b = 0; b < NB; ++b )
+        {
+            const int j = jmap<S>( x.t, b );
+            if ( NBF % TPC == 0 || j < NBF )
+            {
+                const int d = ( j / NS ) * NS * R + ( j % NS );
+#pragma unroll
+                for ( int m = 0; m < R; ++m )
+                    lds_st<P>( x.lds, L( d + m * NS, x.c ), v[b][m] );
+            }
+        }
+    }
+    // inputs of stage S (radix R) from LDS
+    template <int P, int S, int R, int NB>
+    __device__ static __forceinline__ void get_stage( C ( &w )[NB][R], ctx &x )
+    {
+        constexpr int NBF = N / R;
+#pragma unroll
+        for ( int b = 0; b < NB; ++b )
+        {
+            const int j = jmap<S>( x.t, b );
+            if ( NBF % TPC == 0 || j < NBF )
+            {
+#pragma unroll
+                for ( int r = 0; r < R; ++r )
+                    lds_ld<P>( x.lds, L( j + r * NBF, x.c ), w[b][r] );
+            }
+        }
+    }
+    // restaged output elements: element t2 + i TPC of column c2 (and, for R2C, its mirror)
+    template <int P, int NI>
+    __device__ static __forceinline__ void get_restage( C ( &zk )[NI], C ( &zm )[NI], const LW *l, int c2, int t2 )
+    {
+        constexpr int LIM = KIND == KIND_R2C ? N / 2 + 1 : N;
+#pragma unroll
+        for ( int i = 0; i < NI; ++i )
+        {
+            const int k = t2 + i * TPC;
+            if ( LIM % TPC == 0 || k < LIM )
+            {
+                lds_ld<P>( l, L( k, c2 ), zk[i] );
+                if constexpr ( KIND == KIND_R2C )
+                    lds_ld<P>( l, L( k == 0 ? 0 : N - k, c2 ), zm[i] );
+            }
+        }
+    }
 
     // ---- final store of the last stage -------------------------------------------
     template <int R, int NS, int NB>
@@ -330,13 +639,116 @@ struct pass_kernel
     {
         const pass_params<T> &p      = *x.p;
         constexpr int         NBF    = N / R;
-        const bool            direct = ( KIND != KIND_R2C ) && ( p.store_b == (int)LOADB );
-        if ( direct )
+        if constexpr ( FLT )
         {
+            // forward transform of the lines done (natural order: element d + m NS):
+            // multiply by H(k), then run the inverse transform of the same lines from LDS
+            const float k2c = filter_k2( *x.f, x.oa );
 #pragma unroll
             for ( int b = 0; b < NB; ++b )
             {
-                const int j = x.t + TPC * b;
+                const int j = jmap<RP.n - 1>( x.t, b );
+                if ( NBF % TPC == 0 || j < NBF )
+                {
+                    const int d = ( j / NS ) * NS * R + ( j % NS );
+#pragma unroll
+                    for ( int m = 0; m < R; ++m )
+                    {
+                        const int n  = d + m * NS;
+                        const int kk = n <= N / 2 ? n : n - N;
+                        v[b][m]      = scl( v[b][m], filter_h( *x.f, k2c + (float)( kk * kk ) ) );
+                    }
+                }
+            }
+            put_stage<2, RP.n - 1, R, NS, NB>( v, x );
+            __syncthreads();
+            pass_kernel<T, N, KIND_C2C, true, LOADB, VAR, IO_F32, false>::run_from_lds( x );
+            return;
+        }
+        const bool direct = ( KIND != KIND_R2C ) && ( p.store_b == (int)LOADB );
+        if constexpr ( IOM & IO_H16_OUT )
+        {
+            if ( direct )
+            {
+                // one exponent per line: the column's largest magnitude over its threads
+                __shared__ float red[NT];
+                float            lm = 0.f;
+#pragma unroll
+                for ( int b = 0; b < NB; ++b )
+                {
+                    const int j = jmap<RP.n - 1>( x.t, b );
+                    if ( NBF % TPC == 0 || j < NBF )
+                    {
+#pragma unroll
+                        for ( int m = 0; m < R; ++m )
+                            lm = fmaxf( lm, cmag( v[b][m] ) );
+                    }
+                }
+                const int e = h16_exponent( column_max( red, lm, x.c, LOADB ) * fabsf( (float)p.scale ) );
+                h16x2    *dst = static_cast<h16x2 *>( p.out );
+#pragma unroll
+                for ( int b = 0; b < NB; ++b )
+                {
+                    const int j = jmap<RP.n - 1>( x.t, b );
+                    if ( ( NBF % TPC == 0 || j < NBF ) && x.va )
+                    {
+                        const int d = ( j / NS ) * NS * R + ( j % NS );
+#pragma unroll
+                        for ( int m = 0; m < R; ++m )
+                            dst[x.oa + (int64_t)( d + m * NS ) * p.oes] =
+                                h16_pack( v[b][m].x * p.scale, v[b][m].y * p.scale, e );
+                    }
+                }
+                if ( x.t == 0 && x.va )
+                    p.xout[x.ya] = (int8_t)e;
+                return;
+            }
+        }
+        if ( direct )
+        {
+            if constexpr ( PAIR_STORE )
+            {
+                if ( p.vec & 2 )
+                {
+                    // last stage: NS = N / R, so butterfly j writes elements j + m NS, and the
+                    // pair (j, j + 1) writes two consecutive elements per m
+#pragma unroll
+                    for ( int b = 0; b < NB; b += 2 )
+                    {
+                        const int j = jmap<RP.n - 1>( x.t, b );
+                        if ( NBF % TPC == 0 || j < NBF )
+                        {
+#pragma unroll
+                            for ( int m = 0; m < R; ++m )
+                            {
+                                const int64_t n  = j + m * NS;
+                                const C       z0 = v[b][m], z1 = v[b + 1][m];
+                                if constexpr ( KIND == KIND_C2R )
+                                {
+                                    using V2 = typename pair_of<T>::type;
+                                    T *dst   = static_cast<T *>( p.out );
+                                    if ( x.va )
+                                        *reinterpret_cast<V2 *>( dst + x.oa + n ) = V2{ z0.x * p.scale, z1.x * p.scale };
+                                    if ( x.vb )
+                                        *reinterpret_cast<V2 *>( dst + x.ob + n ) = V2{ z0.y * p.scale, z1.y * p.scale };
+                                }
+                                else
+                                {
+                                    C *dst = static_cast<C *>( p.out );
+                                    if ( x.va )
+                                        *reinterpret_cast<f32x4 *>( dst + x.oa + n ) =
+                                            f32x4{ z0.x * p.scale, z0.y * p.scale, z1.x * p.scale, z1.y * p.scale };
+                                }
+                            }
+                        }
+                    }
+                    return;
+                }
+            }
+#pragma unroll
+            for ( int b = 0; b < NB; ++b )
+            {
+                const int j = jmap<RP.n - 1>( x.t, b );
                 if ( NBF % TPC == 0 || j < NBF )
                 {
                     const int d = ( j / NS ) * NS * R + ( j % NS );
@@ -348,22 +760,28 @@ struct pass_kernel
             return;
         }
         // stage the transformed lines through LDS, then store with the output mapping
-#pragma unroll
-        for ( int b = 0; b < NB; ++b )
+        const int     tid = threadIdx.x;
+        const int     c2  = p.store_b ? tid % TW : tid / TPC;
+        const int     t2  = p.store_b ? tid / TW : tid % TPC;
+        constexpr int LIM = KIND == KIND_R2C ? N / 2 + 1 : N;
+        constexpr int NI  = ( LIM + TPC - 1 ) / TPC;
+        C             zk[NI], zm[NI];
+        if constexpr ( SPLIT )
         {
-            const int j = x.t + TPC * b;
-            if ( NBF % TPC == 0 || j < NBF )
-            {
-                const int d = ( j / NS ) * NS * R + ( j % NS );
-#pragma unroll
-                for ( int m = 0; m < R; ++m )
-                    x.lds[L( d + m * NS, x.c )] = v[b][m];
-            }
+            put_stage<0, RP.n - 1, R, NS, NB>( v, x );
+            __syncthreads();
+            get_restage<0, NI>( zk, zm, x.lds, c2, t2 );
+            __syncthreads();
+            put_stage<1, RP.n - 1, R, NS, NB>( v, x );
+            __syncthreads();
+            get_restage<1, NI>( zk, zm, x.lds, c2, t2 );
         }
-        __syncthreads();
-        const int tid = threadIdx.x;
-        const int c2  = p.store_b ? tid % TW : tid / TPC;
-        const int t2  = p.store_b ? tid / TW : tid % TPC;
+        else
+        {
+            put_stage<2, RP.n - 1, R, NS, NB>( v, x );
+            __syncthreads();
+            get_restage<2, NI>( zk, zm, x.lds, c2, t2 );
+        }
         const int64_t col = ( x.tile * TW + c2 ) * CPT;
         const bool    va  = col < p.ncols;
         const bool    vb  = CPT == 2 && col + 1 < p.ncols;
@@ -374,26 +792,113 @@ struct pass_kernel
         if constexpr ( KIND == KIND_R2C )
         {
             // A[k] = (Z[k] + conj Z[N-k]) / 2 ;  B[k] = (Z[k] - conj Z[N-k]) / (2i)
-            T *dst = nullptr;
-            (void)dst;
-            cpx<T> *out = static_cast<cpx<T> *>( p.out );
-            for ( int k = t2; k <= N / 2; k += TPC )
+            C ra[NI], rb[NI];
+#pragma unroll
+            for ( int i = 0; i < NI; ++i )
             {
-                // adjoint of C2R: interior bins of the half spectrum carry weight 2
-                const T h  = ( p.wmode == 1 && k != 0 && 2 * k != N ) ? p.scale : (T)0.5 * p.scale;
-                const C zk = x.lds[L( k, c2 )];
-                const C zm = x.lds[L( k == 0 ? 0 : N - k, c2 )];
-                if ( va )
-                    out[oa + (int64_t)k * p.oes] = mk<T>( h * ( zk.x + zm.x ), h * ( zk.y - zm.y ) );
-                if ( vb )
-                    out[ob + (int64_t)k * p.oes] = mk<T>( h * ( zk.y + zm.y ), h * ( zm.x - zk.x ) );
+                const int k = t2 + i * TPC;
+                if ( LIM % TPC == 0 || k < LIM )
+                {
+                    // adjoint of C2R: interior bins of the half spectrum carry weight 2
+                    const T h = ( p.wmode == 1 && k != 0 && 2 * k != N ) ? p.scale : (T)0.5 * p.scale;
+                    const C a = zk[i], m = zm[i];
+                    ra[i]     = mk<T>( h * ( a.x + m.x ), h * ( a.y - m.y ) );
+                    rb[i]     = mk<T>( h * ( a.y + m.y ), h * ( m.x - a.x ) );
+                }
             }
+            if constexpr ( IOM & IO_H16_OUT )
+            {
+                __shared__ float red[2 * NT];
+                float            ma = 0.f, mb = 0.f;
+#pragma unroll
+                for ( int i = 0; i < NI; ++i )
+                    if ( LIM % TPC == 0 || t2 + i * TPC < LIM )
+                    {
+                        ma = fmaxf( ma, cmag( ra[i] ) );
+                        mb = fmaxf( mb, cmag( rb[i] ) );
+                    }
+                const bool cf = p.store_b != 0;
+                red[NT + tid] = mb;
+                const float cma = column_max( red, ma, c2, cf );
+                float       cmb = 0.f;
+#pragma unroll 8
+                for ( int t = 0; t < TPC; ++t )
+                    cmb = fmaxf( cmb, red[NT + ( cf ? t * TW + c2 : c2 * TPC + t )] );
+                const int ea = h16_exponent( cma ), eb = h16_exponent( cmb );
+                int64_t   xi, ya, yb = 0;
+                col_exponents( p, col, xi, ya );
+                col_exponents( p, col + 1, xi, yb );
+                h16x2 *out = static_cast<h16x2 *>( p.out );
+#pragma unroll
+                for ( int i = 0; i < NI; ++i )
+                {
+                    const int k = t2 + i * TPC;
+                    if ( LIM % TPC == 0 || k < LIM )
+                    {
+                        if ( va )
+                            out[oa + (int64_t)k * p.oes] = h16_pack( ra[i].x, ra[i].y, ea );
+                        if ( vb )
+                            out[ob + (int64_t)k * p.oes] = h16_pack( rb[i].x, rb[i].y, eb );
+                    }
+                }
+                if ( t2 == 0 )
+                {
+                    if ( va )
+                        p.xout[ya] = (int8_t)ea;
+                    if ( vb )
+                        p.xout[yb] = (int8_t)eb;
+                }
+            }
+            else
+            {
+                cpx<T> *out = static_cast<cpx<T> *>( p.out );
+#pragma unroll
+                for ( int i = 0; i < NI; ++i )
+                {
+                    const int k = t2 + i * TPC;
+                    if ( LIM % TPC == 0 || k < LIM )
+                    {
+                        if ( va )
+                            out[oa + (int64_t)k * p.oes] = ra[i];
+                        if ( vb )
+                            out[ob + (int64_t)k * p.oes] = rb[i];
+                    }
+                }
+            }
+        }
+        else if constexpr ( IOM & IO_H16_OUT )
+        {
+            // C2C restaged to the other mapping, FP16 output
+            __shared__ float red[NT];
+            float            lm = 0.f;
+#pragma unroll
+            for ( int i = 0; i < NI; ++i )
+                if ( LIM % TPC == 0 || t2 + i * TPC < LIM )
+                    lm = fmaxf( lm, cmag( zk[i] ) );
+            const int e = h16_exponent( column_max( red, lm, c2, p.store_b != 0 ) * fabsf( (float)p.scale ) );
+            int64_t   xi, ya;
+            col_exponents( p, col, xi, ya );
+            h16x2 *out = static_cast<h16x2 *>( p.out );
+#pragma unroll
+            for ( int i = 0; i < NI; ++i )
+            {
+                const int n = t2 + i * TPC;
+                if ( ( LIM % TPC == 0 || n < LIM ) && va )
+                    out[oa + (int64_t)n * p.oes] = h16_pack( zk[i].x * p.scale, zk[i].y * p.scale, e );
+            }
+            if ( t2 == 0 && va )
+                p.xout[ya] = (int8_t)e;
         }
         else
         {
-#pragma unroll 4
-            for ( int n = t2; n < N; n += TPC )
-                store_line<KIND>( p, oa, ob, va, vb, n, x.lds[L( n, c2 )] );
+            (void)zm;
+#pragma unroll
+            for ( int i = 0; i < NI; ++i )
+            {
+                const int n = t2 + i * TPC;
+                if ( LIM % TPC == 0 || n < LIM )
+                    store_line<KIND>( p, oa, ob, va, vb, n, zk[i] );
+            }
         }
     }
 
@@ -408,39 +913,52 @@ struct pass_kernel
         }
         else
         {
-            constexpr int NBF = N / R;
-#pragma unroll
-            for ( int b = 0; b < NB; ++b )
-            {
-                const int j = x.t + TPC * b;
-                if ( NBF % TPC == 0 || j < NBF )
-                {
-                    const int d = ( j / NS ) * NS * R + ( j % NS );
-#pragma unroll
-                    for ( int m = 0; m < R; ++m )
-                        x.lds[L( d + m * NS, x.c )] = v[b][m];
-                }
-            }
-            __syncthreads();
             constexpr int R2   = RP.r[S + 1];
             constexpr int NS2  = NS * R;
             constexpr int NBF2 = N / R2;
             constexpr int NB2  = ( NBF2 + TPC - 1 ) / TPC;
             constexpr int TST  = N / ( NS2 * R2 );
-            C             w[NB2][R2];
+            // twiddles of the next stage are fetched before the exchange, so their
+            // latency hides behind the LDS writes and the barrier
+            C tws[NB2][R2];
 #pragma unroll
             for ( int b = 0; b < NB2; ++b )
             {
-                const int j = x.t + TPC * b;
+                const int j = jmap<S + 1>( x.t, b );
                 if ( NBF2 % TPC == 0 || j < NBF2 )
                 {
-#pragma unroll
-                    for ( int r = 0; r < R2; ++r )
-                        w[b][r] = x.lds[L( j + r * NBF2, x.c )];
                     const int k = j % NS2;
 #pragma unroll
                     for ( int r = 1; r < R2; ++r )
-                        w[b][r] = twmul<INV>( w[b][r], x.p->tw[r * k * TST] );
+                        tws[b][r] = x.p->tw[r * k * TST];
+                }
+            }
+            C w[NB2][R2];
+            if constexpr ( SPLIT )
+            {
+                put_stage<0, S, R, NS, NB>( v, x );
+                __syncthreads();
+                get_stage<0, S + 1, R2, NB2>( w, x );
+                __syncthreads();
+                put_stage<1, S, R, NS, NB>( v, x );
+                __syncthreads();
+                get_stage<1, S + 1, R2, NB2>( w, x );
+            }
+            else
+            {
+                put_stage<2, S, R, NS, NB>( v, x );
+                __syncthreads();
+                get_stage<2, S + 1, R2, NB2>( w, x );
+            }
+#pragma unroll
+            for ( int b = 0; b < NB2; ++b )
+            {
+                const int j = jmap<S + 1>( x.t, b );
+                if ( NBF2 % TPC == 0 || j < NBF2 )
+                {
+#pragma unroll
+                    for ( int r = 1; r < R2; ++r )
+                        w[b][r] = twmul<INV>( w[b][r], tws[b][r] );
                     dft_fixed<R2, INV>( w[b], x.p->tw, N );
                 }
             }
@@ -449,11 +967,32 @@ struct pass_kernel
         }
     }
 
-    __device__ static __forceinline__ void run( const pass_params<T> &p )
+    // stage 0 from LDS (element n of column c at L(n, c), natural order), then the
+    // remaining stages and the store; used by the inverse half of the filter pass
+    __device__ static __forceinline__ void run_from_lds( ctx &x )
     {
-        __shared__ C lds[LDS];
-        ctx          x;
+        constexpr int R0   = RP.r[0];
+        constexpr int NBF0 = N / R0;
+        constexpr int NB0  = ( NBF0 + TPC - 1 ) / TPC;
+        C             v[NB0][R0];
+        get_stage<2, 0, R0, NB0>( v, x );
+#pragma unroll
+        for ( int b = 0; b < NB0; ++b )
+        {
+            const int j = jmap<0>( x.t, b );
+            if ( NBF0 % TPC == 0 || j < NBF0 )
+                dft_fixed<R0, INV>( v[b], x.p->tw, N );
+        }
+        __syncthreads();
+        chain<0, NB0, R0>( v, x );
+    }
+
+    __device__ static __forceinline__ void run( const pass_params<T> &p, const filter_params<T> *f = nullptr )
+    {
+        __shared__ LW lds[LDS];
+        ctx           x;
         x.p          = &p;
+        x.f          = f;
         x.lds        = lds;
         const int tid = threadIdx.x;
         x.c          = LOADB ? tid % TW : tid / TPC;
@@ -466,32 +1005,97 @@ struct pass_kernel
         col_offsets( p, col, x.ia, x.oa );
         if ( CPT == 2 )
             col_offsets( p, col + 1, x.ib, x.ob );
+        x.xa = x.xb = x.ya = x.yb = 0;
+        if constexpr ( IOM != IO_F32 )
+        {
+            col_exponents( p, col, x.xa, x.ya );
+            if ( CPT == 2 )
+                col_exponents( p, col + 1, x.xb, x.yb );
+        }
 
         constexpr int R0   = RP.r[0];
         constexpr int NBF0 = N / R0;
         constexpr int NB0  = ( NBF0 + TPC - 1 ) / TPC;
         C             v[NB0][R0];
-#pragma unroll
-        for ( int b = 0; b < NB0; ++b )
+        bool          loaded = false;
+        if constexpr ( PAIR_LOAD )
         {
-            const int j = x.t + TPC * b;
-            if ( NBF0 % TPC == 0 || j < NBF0 )
+            if ( p.vec & 1 )
             {
+                // butterflies j, j + 1 read elements j + r NBF0 and j + 1 + r NBF0: one
+                // 8-byte (two reals) or 16-byte (two FP32 complex) load per line and r
 #pragma unroll
-                for ( int r = 0; r < R0; ++r )
-                    v[b][r] = load_line<KIND>( p, x.ia, x.ib, x.va, x.vb, j + r * NBF0, N );
-                dft_fixed<R0, INV>( v[b], p.tw, N );
+                for ( int b = 0; b < NB0; b += 2 )
+                {
+                    const int j = jmap<0>( x.t, b );
+                    if ( NBF0 % TPC == 0 || j < NBF0 )
+                    {
+#pragma unroll
+                        for ( int r = 0; r < R0; ++r )
+                        {
+                            const int64_t n = j + r * NBF0;
+                            if constexpr ( KIND == KIND_R2C )
+                            {
+                                using V2      = typename pair_of<T>::type;
+                                const T *src  = static_cast<const T *>( p.in );
+                                const V2 a    = x.va ? *reinterpret_cast<const V2 *>( src + x.ia + n ) : V2{ 0, 0 };
+                                const V2 bq   = x.vb ? *reinterpret_cast<const V2 *>( src + x.ib + n ) : V2{ 0, 0 };
+                                v[b][r]       = mk<T>( a.x, bq.x );
+                                v[b + 1][r]   = mk<T>( a.y, bq.y );
+                            }
+                            else
+                            {
+                                const C    *src = static_cast<const C *>( p.in );
+                                const f32x4 q   = x.va ? *reinterpret_cast<const f32x4 *>( src + x.ia + n )
+                                                       : f32x4{ 0, 0, 0, 0 };
+                                v[b][r]         = mk<T>( q.x, q.y );
+                                v[b + 1][r]     = mk<T>( q.z, q.w );
+                            }
+                        }
+                        dft_fixed<R0, INV>( v[b], p.tw, N );
+                        dft_fixed<R0, INV>( v[b + 1], p.tw, N );
+                    }
+                }
+                loaded = true;
+            }
+        }
+        if ( !loaded )
+        {
+#pragma unroll
+            for ( int b = 0; b < NB0; ++b )
+            {
+                const int j = jmap<0>( x.t, b );
+                if ( NBF0 % TPC == 0 || j < NBF0 )
+                {
+#pragma unroll
+                    for ( int r = 0; r < R0; ++r )
+                    {
+                        if constexpr ( IOM & IO_H16_IN )
+                            v[b][r] = load_line_h16<KIND>( p, x.ia, x.ib, x.xa, x.xb, x.va, x.vb, j + r * NBF0, N );
+                        else
+                            v[b][r] = load_line<KIND>( p, x.ia, x.ib, x.va, x.vb, j + r * NBF0, N );
+                    }
+                    dft_fixed<R0, INV>( v[b], p.tw, N );
+                }
             }
         }
         chain<0, NB0, R0>( v, x );
     }
 };
 
-template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR>
-__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB, VAR>::NT ) )
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32>
+__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::NT ) )
     k_pass( const pass_params<T> p )
 {
-    pass_kernel<T, N, KIND, INV, LOADB, VAR>::run( p );
+    pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::run( p );
+}
+
+// fused filter pass: forward C2C along the lines, x H(k), inverse C2C, one HBM round trip
+template <typename T, int N, bool LOADB, int VAR>
+__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND_C2C, false, LOADB, VAR, IO_F32, true>::NT ) )
+    k_pass_filter( const pass_params<T> p, const filter_params<T> f )
+{
+    pass_kernel<T, N, KIND_C2C, false, LOADB, VAR, IO_F32, true>::run( p, &f );
 }
 
 // ---------------------------------------------------------------------------

@@ -40,13 +40,16 @@ template <int NA, int NB, int CS>
 struct plane_geom
 {
     static constexpr int NAH   = NA / 2 + 1;
-    static constexpr int PA    = NAH;                 // pitch of S rows (complex)
-    static constexpr int PZ    = NA;                  // pitch of Z rows (complex)
+    static constexpr int PA    = NAH;                   // pitch of S rows (complex)
+    static constexpr int ZSH   = 4;                     // Z rows: one pad element every 16
+    static constexpr int PZ    = NA + ( NA >> ZSH );    // pitch of Z rows (complex)
     static constexpr int ZELEM = ( NB / 2 ) * PZ;
     static constexpr int SELEM = NB * PA;
     static constexpr int ELEMS = ZELEM > SELEM ? ZELEM : SELEM;
-    static constexpr int BYTES = ELEMS * CS;
-    static constexpr int NT    = BYTES > 32768 ? 512 : 256;
+    static constexpr int TWOFF = ELEMS;                 // twiddle tables (NA, then NB entries)
+    static constexpr int TOTAL = ELEMS + NA + NB;
+    static constexpr int BYTES = TOTAL * CS;
+    static constexpr int NT    = ELEMS * CS > 32768 ? 512 : 256;
 };
 
 constexpr int plane_lds_limit() { return 163840; }
@@ -54,9 +57,11 @@ constexpr int plane_lds_limit() { return 163840; }
 // In-LDS Stockham transform of NL lines of length N.  ROWS: the line elements are
 // contiguous (element stride 1, line stride LS) and lanes walk along a line; otherwise
 // lanes walk across lines (line stride 1, element stride ES).
-template <typename T, int N, int NL, bool ROWS, bool INV, int NT, int S = 0>
-__device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, const cpx<T> *__restrict__ tw )
+template <typename T, int N, int NL, bool ROWS, bool INV, int NT, int PSH, int S = 0>
+__device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, const cpx<T> *tw )
 {
+    // element e of a line sits at e + (e >> PSH) (PSH = 0: unpadded)
+    auto at = [&]( int line, int e ) { return line * ls + ( PSH ? e + ( e >> PSH ) : e ) * es; };
     constexpr rplan RP = make_rplan( N );
     if constexpr ( S < RP.n )
     {
@@ -76,10 +81,9 @@ __device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, const cp
             {
                 const int line = ROWS ? item / NBF : item % NL;
                 const int j    = ROWS ? item % NBF : item / NL;
-                const int base = line * ls;
 #pragma unroll
                 for ( int r = 0; r < R; ++r )
-                    v[i][r] = buf[base + ( j + r * NBF ) * es];
+                    v[i][r] = buf[at( line, j + r * NBF )];
                 const int k = j % NS;
                 if constexpr ( NS > 1 )
                 {
@@ -102,11 +106,11 @@ __device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, const cp
                 const int d    = ( j / NS ) * NS * R + ( j % NS );
 #pragma unroll
                 for ( int m = 0; m < R; ++m )
-                    buf[line * ls + ( d + m * NS ) * es] = v[i][m];
+                    buf[at( line, d + m * NS )] = v[i][m];
             }
         }
         __syncthreads();
-        lds_lines<T, N, NL, ROWS, INV, NT, S + 1>( buf, es, ls, tw );
+        lds_lines<T, N, NL, ROWS, INV, NT, PSH, S + 1>( buf, es, ls, tw );
     }
 }
 
@@ -120,6 +124,8 @@ __device__ __forceinline__ void plane_offsets( const plane_params<T> &p, int64_t
 }
 
 // FWD: real -> spectrum (R2C along A, C2C forward along B); else the inverse.
+// Every HBM phase is unrolled so that a thread issues all of its loads before the
+// first LDS write; the twiddle tables are copied to LDS with the first loads.
 template <typename T, int NA, int NB, bool FWD>
 __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::NT ) )
     k_plane( const plane_params<T> p )
@@ -130,39 +136,60 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
     constexpr int NAH       = G::NAH;
     constexpr int PA        = G::PA;
     constexpr int PZ        = G::PZ;
+    constexpr int ZSH       = G::ZSH;
     constexpr int NP        = NB / 2;
     static_assert( NB % 2 == 0, "plane pass pairs the rows of B" );
     static_assert( G::BYTES <= plane_lds_limit(), "plane does not fit in LDS" );
-    __shared__ C  lds[G::ELEMS];
+    __shared__ C  lds[G::TOTAL];
+    C            *twa   = lds + G::TWOFF;
+    C            *twb   = twa + NA;
     const int     tid   = threadIdx.x;
     const int64_t plane = (int64_t)xcd_tile( blockIdx.x, gridDim.x );
     int64_t       ro, co;
     plane_offsets( p, plane, ro, co );
+    auto zi = []( int q, int n ) { return q * PZ + n + ( n >> ZSH ); }; // Z[q][n]
+
+    // twiddle tables into LDS (read once per workgroup, from L2)
+    for ( int e = tid; e < NA + NB; e += NT )
+        twa[e] = e < NA ? p.twa[e] : p.twb[e - NA];
+
+    constexpr int ITZ = NP * NA;  // Z elements (pairs of reals)
+    constexpr int PZT = ( ITZ + NT - 1 ) / NT;
+    constexpr int ITS = NB * NAH; // S elements (half-spectrum values)
+    constexpr int PST = ( ITS + NT - 1 ) / NT;
 
     if constexpr ( FWD )
     {
         const T *src = static_cast<const T *>( p.in ) + ro;
         // 1. Z[q][n] = (x[2q][n], x[2q+1][n])
-        if ( p.ra == 1 )
+        T ua[PZT], ub[PZT];
+#pragma unroll
+        for ( int i = 0; i < PZT; ++i )
         {
-            for ( int e = tid; e < NP * NA; e += NT )
+            const int e = tid + i * NT;
+            if ( ITZ % NT == 0 || e < ITZ )
             {
-                const int q = e / NA, n = e % NA;
-                lds[q * PZ + n] = mk<T>( src[(int64_t)( 2 * q ) * p.rb + n], src[(int64_t)( 2 * q + 1 ) * p.rb + n] );
+                const int     q   = p.ra == 1 ? e / NA : e % NP;
+                const int     n   = p.ra == 1 ? e % NA : e / NP;
+                const int64_t off = (int64_t)n * p.ra + (int64_t)( 2 * q ) * p.rb;
+                ua[i]             = src[off];
+                ub[i]             = src[off + p.rb];
             }
         }
-        else
+#pragma unroll
+        for ( int i = 0; i < PZT; ++i )
         {
-            for ( int e = tid; e < NP * NA; e += NT )
+            const int e = tid + i * NT;
+            if ( ITZ % NT == 0 || e < ITZ )
             {
-                const int q = e % NP, n = e / NP;
-                lds[q * PZ + n] = mk<T>( src[(int64_t)n * p.ra + ( 2 * q ) * p.rb],
-                                         src[(int64_t)n * p.ra + ( 2 * q + 1 ) * p.rb] );
+                const int q = p.ra == 1 ? e / NA : e % NP;
+                const int n = p.ra == 1 ? e % NA : e / NP;
+                lds[zi( q, n )] = mk<T>( ua[i], ub[i] );
             }
         }
         __syncthreads();
         // 2. rows of Z
-        lds_lines<T, NA, NP, true, false, NT>( lds, 1, PZ, p.twa );
+        lds_lines<T, NA, NP, true, false, NT, ZSH>( lds, 1, PZ, twa );
         // 3. separate pairs: S[2q][k] = A, S[2q+1][k] = B   (S aliases Z)
         constexpr int ITEMS = NP * NAH;
         constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
@@ -174,8 +201,8 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
             if ( ITEMS % NT == 0 || e < ITEMS )
             {
                 const int q = e / NAH, k = e % NAH;
-                const C   zk = lds[q * PZ + k];
-                const C   zm = lds[q * PZ + ( k == 0 ? 0 : NA - k )];
+                const C   zk = lds[zi( q, k )];
+                const C   zm = lds[zi( q, k == 0 ? 0 : NA - k )];
                 const T   h  = ( p.wmode == 1 && k != 0 && 2 * k != NA ) ? (T)1 : (T)0.5;
                 a[i]         = mk<T>( h * ( zk.x + zm.x ), h * ( zk.y - zm.y ) );
                 b[i]         = mk<T>( h * ( zk.y + zm.y ), h * ( zm.x - zk.x ) );
@@ -195,22 +222,17 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
         }
         __syncthreads();
         // 4. columns of S
-        lds_lines<T, NB, NAH, false, false, NT>( lds, PA, 1, p.twb );
+        lds_lines<T, NB, NAH, false, false, NT, 0>( lds, PA, 1, twb );
         // 5. store
         C *dst = static_cast<C *>( p.out ) + co;
-        if ( p.ca == 1 )
+#pragma unroll
+        for ( int i = 0; i < PST; ++i )
         {
-            for ( int e = tid; e < NB * NAH; e += NT )
+            const int e = tid + i * NT;
+            if ( ITS % NT == 0 || e < ITS )
             {
-                const int bb = e / NAH, k = e % NAH;
-                dst[(int64_t)bb * p.cb + k] = scl( lds[bb * PA + k], p.scale );
-            }
-        }
-        else
-        {
-            for ( int e = tid; e < NB * NAH; e += NT )
-            {
-                const int bb = e % NB, k = e / NB;
+                const int bb = p.ca == 1 ? e / NAH : e % NB;
+                const int k  = p.ca == 1 ? e % NAH : e / NB;
                 dst[(int64_t)k * p.ca + (int64_t)bb * p.cb] = scl( lds[bb * PA + k], p.scale );
             }
         }
@@ -219,34 +241,39 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
     {
         const C *src = static_cast<const C *>( p.in ) + co;
         // 1. load S
-        if ( p.ca == 1 )
+        C u[PST];
+#pragma unroll
+        for ( int i = 0; i < PST; ++i )
         {
-            for ( int e = tid; e < NB * NAH; e += NT )
+            const int e = tid + i * NT;
+            if ( ITS % NT == 0 || e < ITS )
             {
-                const int bb = e / NAH, k = e % NAH;
-                lds[bb * PA + k] = src[(int64_t)bb * p.cb + k];
+                const int bb = p.ca == 1 ? e / NAH : e % NB;
+                const int k  = p.ca == 1 ? e % NAH : e / NB;
+                u[i]         = src[(int64_t)k * p.ca + (int64_t)bb * p.cb];
             }
         }
-        else
+#pragma unroll
+        for ( int i = 0; i < PST; ++i )
         {
-            for ( int e = tid; e < NB * NAH; e += NT )
+            const int e = tid + i * NT;
+            if ( ITS % NT == 0 || e < ITS )
             {
-                const int bb = e % NB, k = e / NB;
-                lds[bb * PA + k] = src[(int64_t)k * p.ca + (int64_t)bb * p.cb];
+                const int bb = p.ca == 1 ? e / NAH : e % NB;
+                const int k  = p.ca == 1 ? e % NAH : e / NB;
+                lds[bb * PA + k] = u[i];
             }
         }
         __syncthreads();
         // 2. columns of S, inverse
-        lds_lines<T, NB, NAH, false, true, NT>( lds, PA, 1, p.twb );
+        lds_lines<T, NB, NAH, false, true, NT, 0>( lds, PA, 1, twb );
         // 3. Z[q][n] = A~[n] + i B~[n] from rows 2q, 2q+1 (Hermitian extension)
-        constexpr int ITEMS = NP * NA;
-        constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
-        C             z[PER];
+        C z[PZT];
 #pragma unroll
-        for ( int i = 0; i < PER; ++i )
+        for ( int i = 0; i < PZT; ++i )
         {
             const int e = tid + i * NT;
-            if ( ITEMS % NT == 0 || e < ITEMS )
+            if ( ITZ % NT == 0 || e < ITZ )
             {
                 const int q = e / NA, n = e % NA;
                 const int nn = n <= NA / 2 ? n : NA - n;
@@ -272,35 +299,29 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
         }
         __syncthreads();
 #pragma unroll
-        for ( int i = 0; i < PER; ++i )
+        for ( int i = 0; i < PZT; ++i )
         {
             const int e = tid + i * NT;
-            if ( ITEMS % NT == 0 || e < ITEMS )
-                lds[( e / NA ) * PZ + ( e % NA )] = z[i];
+            if ( ITZ % NT == 0 || e < ITZ )
+                lds[zi( e / NA, e % NA )] = z[i];
         }
         __syncthreads();
         // 4. rows of Z, inverse
-        lds_lines<T, NA, NP, true, true, NT>( lds, 1, PZ, p.twa );
+        lds_lines<T, NA, NP, true, true, NT, ZSH>( lds, 1, PZ, twa );
         // 5. store the real rows
         T *dst = static_cast<T *>( p.out ) + ro;
-        if ( p.ra == 1 )
+#pragma unroll
+        for ( int i = 0; i < PZT; ++i )
         {
-            for ( int e = tid; e < NP * NA; e += NT )
+            const int e = tid + i * NT;
+            if ( ITZ % NT == 0 || e < ITZ )
             {
-                const int q = e / NA, n = e % NA;
-                const C   v = lds[q * PZ + n];
-                dst[(int64_t)( 2 * q ) * p.rb + n]     = v.x * p.scale;
-                dst[(int64_t)( 2 * q + 1 ) * p.rb + n] = v.y * p.scale;
-            }
-        }
-        else
-        {
-            for ( int e = tid; e < NP * NA; e += NT )
-            {
-                const int q = e % NP, n = e / NP;
-                const C   v = lds[q * PZ + n];
-                dst[(int64_t)n * p.ra + ( 2 * q ) * p.rb]     = v.x * p.scale;
-                dst[(int64_t)n * p.ra + ( 2 * q + 1 ) * p.rb] = v.y * p.scale;
+                const int     q   = p.ra == 1 ? e / NA : e % NP;
+                const int     n   = p.ra == 1 ? e % NA : e / NP;
+                const C       v   = lds[zi( q, n )];
+                const int64_t off = (int64_t)n * p.ra + (int64_t)( 2 * q ) * p.rb;
+                dst[off]          = v.x * p.scale;
+                dst[off + p.rb]   = v.y * p.scale;
             }
         }
     }

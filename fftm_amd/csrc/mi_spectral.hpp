@@ -10,6 +10,7 @@
 #pragma once
 
 #include "mi_complex.hpp"
+#include "mi_pass.hpp"
 
 namespace fftm_mi
 {
@@ -53,6 +54,16 @@ __global__ void __launch_bounds__( 256 )
         const T h = filter ? filter[i] : coeffs[spec_shell( g, i )];
         out[i]    = scl( in[i], h );
     }
+}
+
+// out[i] = H(k_i) in[i] for an analytic operator (Poisson, Gaussian, radial shells) on
+// the dense half-spectrum layout described by f
+template <typename T>
+__global__ void __launch_bounds__( 256 )
+    k_spectral_op( const filter_params<T> f, const cpx<T> *in, cpx<T> *out, int64_t total )
+{
+    for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256 )
+        out[i] = scl( in[i], filter_h( f, filter_k2( f, i ) ) );
 }
 
 // partial[b * S + s] = sum over this block's bins in shell s of Re(conj(G_i) X_i), in FP64

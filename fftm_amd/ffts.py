@@ -94,6 +94,10 @@ class ffts_init_options:
     execution: str = "fused"          # "fused" | "staged"
     sync: bool = True                 # device-synchronise after each call (reference safe-call behaviour)
     destructive_backward: bool = False
+    # "fp32": spectra between passes in the transform precision; "fp16": FP32 transforms
+    # whose intermediate half spectra are stored in FP16 with per-line power-of-two
+    # exponents (3D, fused execution, lengths 128..2048)
+    spectrum_storage: str = "fp32"
 
 
 class _profiler:
@@ -178,6 +182,8 @@ class ffts:
         for d in dims:
             if int(d) < 1:
                 raise ValueError("ffts::init: sizes must be positive")
+        if self._options.spectrum_storage == "fp16" and (dim != 3 or self._options.execution != "fused"):
+            raise ValueError("ffts: FP16 spectrum storage is implemented for fused 3D transforms")
         with self._profiler.scope(f"ffts::init<{dim}>"):
             self.nx, self.ny = int(nx), int(ny)
             self.nz = int(nz) if nz is not None else 0
@@ -220,6 +226,12 @@ class ffts:
         import torch
         L = _lib.lib()
         prec = _lib.F32 if self.real == torch.float32 else _lib.F64
+        if self._options.spectrum_storage == "fp16":
+            if self.real != torch.float32:
+                raise TypeError("ffts: FP16 spectrum storage needs float32 transforms")
+            prec = _lib.F32_H16
+        elif self._options.spectrum_storage != "fp32":
+            raise ValueError("ffts: spectrum_storage is 'fp32' or 'fp16'")
         flags = _lib.FLAG_OVERWRITE_INPUT if self._options.destructive_backward else 0
         h = ctypes.c_void_p()
         _lib.check(L.fftm_mi_ffts_create(layout, _lib.i64_array(n), prec, flags, ctypes.byref(h)),
@@ -491,6 +503,46 @@ class ffts:
             self._handle, device_ptr(inp), device_ptr(out), device_ptr(filter) if filter is not None else None,
             device_ptr(shell_coeffs) if shell_coeffs is not None else None, ns, 1 if self._options.sync else 0),
             "fftm_mi_ffts_spectral_multiply")
+
+    def _op(self, kind, scale, sigma, coeffs):
+        kinds = {"poisson": _lib.OP_POISSON, "gaussian": _lib.OP_GAUSSIAN, "shell": _lib.OP_SHELL}
+        if kind not in kinds:
+            raise ValueError(f"spectral operator kind must be one of {sorted(kinds)}")
+        op = _lib.SpectralOp()
+        op.kind = kinds[kind]
+        op.scale = float(scale)
+        op.sigma = float(sigma or 0.0)
+        op.nshells = 0
+        op.coeffs = None
+        if kind == "shell":
+            if coeffs is None or coeffs.dtype != self.real or not coeffs.is_cuda or coeffs.dim() != 1:
+                raise ValueError("shell operator: coeffs must be a 1D GPU tensor of the real type")
+            op.nshells = coeffs.numel()
+            op.coeffs = device_ptr(coeffs)
+        return op
+
+    def spectral_apply(self, inp, out, kind, scale=1.0, sigma=None, coeffs=None):
+        """out = H(k) * inp with an analytic operator: "poisson" (-scale/|k|^2, 0 at k = 0),
+        "gaussian" (scale * exp(-|k|^2 / (2 sigma^2))) or "shell" (scale * coeffs[shell(k)])."""
+        self._need_handle("spectral_apply")
+        self._check(inp, self.complex_shape(), self.complex, "spectral_apply input")
+        self._check(out, self.complex_shape(), self.complex, "spectral_apply output")
+        op = self._op(kind, scale, sigma, coeffs)
+        _lib.check(_lib.lib().fftm_mi_ffts_spectral_apply(self._handle, device_ptr(inp), device_ptr(out),
+                                                          ctypes.byref(op), 1 if self._options.sync else 0),
+                   "fftm_mi_ffts_spectral_apply")
+
+    def filter(self, inp, out, kind, scale=1.0, sigma=None, coeffs=None):
+        """out = backward(H * forward(inp)) in one call (the last forward axis, the multiply and
+        the first backward axis fused into one pass); scale folds in the 1/N normalisation."""
+        self._need_handle("filter")
+        self._check(inp, self.real_shape(), self.real, "filter input")
+        self._check(out, self.real_shape(), self.real, "filter output")
+        op = self._op(kind, scale, sigma, coeffs)
+        with self._profiler.scope("ffts::filter"):
+            _lib.check(_lib.lib().fftm_mi_ffts_filter(self._handle, device_ptr(inp), device_ptr(out),
+                                                      ctypes.byref(op), 1 if self._options.sync else 0),
+                       "fftm_mi_ffts_filter")
 
     def shell_gradient(self, grad_spectrum, spectrum, nshells):
         """g[s] = sum_{k in shell s} Re(conj(G_k) X_k)."""

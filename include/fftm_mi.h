@@ -52,6 +52,11 @@ extern "C" {
 /* --- precisions ----------------------------------------------------------- */
 #define FFTM_MI_F32 0 /* float real / float2 complex (hipfftReal / hipfftComplex)          */
 #define FFTM_MI_F64 1 /* double real / double2 complex (hipfftDoubleReal / ...Complex)     */
+/* FP32 arrays and arithmetic; the intermediate half spectra between the passes of an
+ * ffts transform are stored in FP16 with one power-of-two exponent per stored line
+ * (4 bytes per bin instead of 8).  ffts 3D layout only; lengths 128..2048 (powers of
+ * two).  No counterpart in the reference.                                             */
+#define FFTM_MI_F32_H16 2
 
 /* --- plan flags ----------------------------------------------------------- */
 /* C2R of rank >= 2 keeps its input intact by default and needs a work area of one
@@ -168,11 +173,12 @@ int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *b
  * passed; the pass reads and writes exactly what it does inside that call.  Used to
  * time individual kernels; it has no counterpart in the reference. */
 int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
-/* Tile-shape variant (0..5) of one pass's kernel: 0 default, 1 half the columns per
+/* Tile-shape variant (0..7) of one pass's kernel: 0 default, 1 half the columns per
  * workgroup, 2 twice the threads per column, 3 both, 4 twice the columns (256-byte row
- * segments, up to 160 KB of LDS), 5 twice the columns and the threads per column.
- * Results are bitwise identical across variants; only the speed differs.  Set by the
- * tuner or by hand. */
+ * segments, up to 160 KB of LDS), 5 twice the columns and the threads per column,
+ * 6 and 7 the tiles of 0 and 4 exchanged through LDS one component at a time (half
+ * the LDS per workgroup).  Results are bitwise identical across variants; only the
+ * speed differs.  Set by the tuner or by hand. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
 int fftm_mi_ffts_get_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t *variant );
 /* Algorithmic HBM bytes one pass moves (reads + writes of its input and output). */
@@ -204,6 +210,43 @@ int fftm_mi_ffts_c2r_adjoint( fftm_mi_ffts h, const void *real_in, void *spectru
  */
 int fftm_mi_ffts_spectral_multiply( fftm_mi_ffts h, const void *in, void *out, const void *filter,
                                     const void *shell_coeffs, int32_t nshells, int32_t sync );
+
+/*
+ * Analytic diagonal operators on the half spectrum (k as for the shell filter):
+ *   FFTM_MI_OP_POISSON   H = -scale / |k|^2, H(0) = 0   (periodic inverse Laplacian,
+ *                        the solve_poisson_3d_functor of
+ *                        source/tests/detail/versioned_ffts_3d_test_driver.h:128-154)
+ *   FFTM_MI_OP_GAUSSIAN  H = scale * exp(-|k|^2 / (2 sigma^2))
+ *   FFTM_MI_OP_SHELL     H = scale * coeffs[shell(k)]   (coeffs: device T[nshells])
+ */
+#define FFTM_MI_OP_POISSON 1
+#define FFTM_MI_OP_GAUSSIAN 2
+#define FFTM_MI_OP_SHELL 3
+typedef struct fftm_mi_spectral_op
+{
+    int32_t     kind;
+    int32_t     nshells; /* OP_SHELL: 1..1024                                   */
+    double      scale;   /* overall factor, e.g. 1/N to normalise a round trip   */
+    double      sigma;   /* OP_GAUSSIAN: width in wavenumber units (> 0)         */
+    const void *coeffs;  /* OP_SHELL: device array of nshells values of type T  */
+} fftm_mi_spectral_op;
+
+/* spectrum_out = H spectrum_in on the handle's spectrum layout (in == out allowed). */
+int fftm_mi_ffts_spectral_apply( fftm_mi_ffts h, const void *spectrum_in, void *spectrum_out,
+                                 const fftm_mi_spectral_op *op, int32_t sync );
+
+/*
+ * real_out = backward(H forward(real_in)) in one call, the spectrum kept in the work
+ * area.  The last forward pass, the multiply and the first backward pass run as one
+ * kernel along the same lines (one HBM round trip instead of three), and op->scale
+ * absorbs the 1/N normalisation, so a periodic Poisson solve or a spectral low-pass is
+ * five passes over HBM in 3D.  Replaces the reference's forward / functor / backward /
+ * scale sequence (versioned_ffts_3d_test_driver.h:640-655).  Needs the work area (a
+ * handle created without FFTM_MI_FLAG_OVERWRITE_INPUT); not for FFTM_MI_F32_H16.
+ * real_in == real_out is allowed.
+ */
+int fftm_mi_ffts_filter( fftm_mi_ffts h, const void *real_in, void *real_out, const fftm_mi_spectral_op *op,
+                         int32_t sync );
 
 /* grad[s] = sum_{k in shell s} Re(conj(G_k) X_k) (gradient of a loss w.r.t. the shell
  * coefficients, G = dL/dZ for Z = H X), accumulated in FP64, written as T[nshells]. */

@@ -449,3 +449,37 @@ def realaxis_weights_logical(dims) -> np.ndarray:
     if n % 2 == 0:
         w[-1] = 1.0
     return np.broadcast_to(w, tuple(dims[:-1]) + (n // 2 + 1,))
+
+
+def spectral_operator_logical(dims, kind: str, scale: float = 1.0, sigma: float = 0.0, coeffs=None) -> np.ndarray:
+    """H on the logical half spectrum (last axis halved), FP64:
+    poisson  -scale/|k|^2 with H(0) = 0 (solve_poisson_3d_functor,
+             versioned_ffts_3d_test_driver.h:128-154),
+    gaussian scale*exp(-|k|^2/(2 sigma^2)),
+    shell    scale*coeffs[shell(k)].
+    Pinned in tests by closed forms: the periodic Poisson problem of
+    versioned_ffts_3d_test_driver.h:187-209 and Gaussian-filtered plane waves."""
+    d = len(dims)
+    ks = [wavenumbers(n, a == d - 1) for a, n in enumerate(dims)]
+    grids = np.meshgrid(*ks, indexing="ij")
+    k2 = np.zeros(grids[0].shape, dtype=np.float64)
+    for g in grids:
+        k2 += g.astype(np.float64) ** 2
+    if kind == "poisson":
+        with np.errstate(divide="ignore"):
+            h = np.where(k2 > 0, -scale / np.where(k2 > 0, k2, 1.0), 0.0)
+        return h
+    if kind == "gaussian":
+        return scale * np.exp(-k2 / (2.0 * sigma * sigma))
+    if kind == "shell":
+        c = np.asarray(coeffs, dtype=np.float64)
+        return scale * c[shell_index_logical(dims, len(c))]
+    raise ValueError(kind)
+
+
+def filter_logical(x: np.ndarray, h: np.ndarray) -> np.ndarray:
+    """backward(H * forward(x)) on a logical real array, unnormalised like the ffts
+    backward (x N): N * irfftn(H * rfftn(x)) over all axes, last axis halved, in FP64."""
+    axes = tuple(range(x.ndim))
+    X = np.fft.rfftn(x.astype(np.float64), axes=axes)
+    return np.fft.irfftn(X * h, s=x.shape, axes=axes) * float(np.prod(x.shape))

@@ -199,3 +199,18 @@ def test_hardware_signature_needs_device():
         assert rc == 11
     else:
         assert rc == 0 and buf.value.startswith(b"v2:")
+
+
+def test_ffts_validate_fp16_storage():
+    L = _lib()
+    ws = ctypes.c_size_t()
+    al = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    n0, n1, n2 = 256, 128, 512
+    nzh = n2 // 2 + 1
+    rc = L.lib().fftm_mi_ffts_validate(0, L.i64_array([n0, n1, n2]), L.F32_H16, 0, ctypes.byref(ws))
+    assert rc == 0, L.last_error()
+    assert ws.value == al(4 * n0 * n1 * nzh) + al(max(n0 * n1, n1 * nzh)) + al(n0 * nzh)
+    # 4D layouts and lengths outside 128..2048 are not implemented for FP16 storage
+    assert L.lib().fftm_mi_ffts_validate(1, L.i64_array([128] * 4), L.F32_H16, 0, ctypes.byref(ws)) == 16
+    assert L.lib().fftm_mi_ffts_validate(0, L.i64_array([128, 96, 128]), L.F32_H16, 0, ctypes.byref(ws)) == 16
+    assert L.lib().fftm_mi_ffts_validate(0, L.i64_array([64, 128, 128]), L.F32_H16, 0, ctypes.byref(ws)) == 16

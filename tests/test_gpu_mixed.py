@@ -1,0 +1,133 @@
+"""FP32 transforms with FP16-stored intermediate spectra (no reference counterpart).
+
+The half spectrum between passes is stored as FP16 pairs with one power-of-two
+exponent per stored line, chosen so that the line's largest component lands in
+[2^14, 2^15).  Each storage rounds with unit roundoff 2^-11, so the forward spectrum
+(two roundings) stays within a few 1e-4 relative L2 of the FP64 oracle; the plain
+FP32 path is the accuracy judge.  Scaling the input by a power of two scales every
+stored exponent by the same amount, so results scale bitwise exactly -- that pins the
+exponent bookkeeping independently of the FP16 rounding."""
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import from_gpu
+
+pytestmark = pytest.mark.gpu
+
+# FP16 storage: 2 roundings (forward) / 4 (round trip) of unit roundoff 2^-11 each
+TOL_FWD_L2 = 1.5e-3
+TOL_RT_L2 = 3e-3
+
+
+def _ffts(dims, storage="fp16"):
+    import fftm_amd
+    f = fftm_amd.ffts("float32")
+    f.init(*dims, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="",
+                                                     spectrum_storage=storage))
+    return f
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128), (256, 128, 512), (128, 256, 256), (2048, 128, 128)])
+def test_h16_matches_oracle(gpu, dims):
+    import torch
+    print(f"case h16 3d {dims}", flush=True)
+    x = oracle.fill_random_real_3d(*dims, 0x1234ABCDEF987654, np.float32)
+    f = _ffts(dims)
+    u = torch.from_numpy(x).cuda()
+    X = f.complex_array()
+    f.forward(u, X)
+    Xo = oracle.ffts3d_forward(x.astype(np.float64))
+    l2, mx = oracle.rel_errors(from_gpu(X), Xo)
+    print(f"[h16] fwd {dims}: rel_l2={l2:.3e} rel_max={mx:.3e}", flush=True)
+    assert l2 < TOL_FWD_L2 and mx < 4 * TOL_FWD_L2
+    y = f.real_array()
+    f.backward(X, y)
+    yr = from_gpu(y).astype(np.float64) / np.prod(dims)
+    rt = float(np.linalg.norm(yr - x) / np.linalg.norm(x))
+    print(f"[h16] round trip {dims}: rel_l2={rt:.3e}", flush=True)
+    assert rt < TOL_RT_L2
+    # the plain FP32 path on the same input is an order of magnitude more accurate
+    g = _ffts(dims, "fp32")
+    X32 = g.complex_array()
+    g.forward(u, X32)
+    l2_32, _ = oracle.rel_errors(from_gpu(X32), Xo)
+    assert l2_32 < l2 / 10
+    f.close()
+    g.close()
+
+
+@pytest.mark.parametrize("shift", [20, -30])
+def test_h16_power_of_two_scaling_is_exact(gpu, shift):
+    """x * 2^s gives spectrum * 2^s bitwise: the per-line exponents absorb the scale
+    (2^-30 puts every value below the FP16 normal range without them)."""
+    import torch
+    dims = (128, 128, 256)
+    x = oracle.fill_random_real_3d(*dims, 77, np.float32)
+    f = _ffts(dims)
+    u = torch.from_numpy(x).cuda()
+    us = u * (2.0 ** shift)
+    X, Xs = f.complex_array(), f.complex_array()
+    f.forward(u, X)
+    f.forward(us, Xs)
+    assert torch.equal(Xs, X * (2.0 ** shift))
+    y, ys = f.real_array(), f.real_array()
+    f.backward(X, y)
+    f.backward(Xs, ys)
+    assert torch.equal(ys, y * (2.0 ** shift))
+    f.close()
+
+
+def test_h16_wide_dynamic_range_spectrum(gpu):
+    """uniform(-1,1) plus a k^-5/3 random-phase field: the per-line exponents keep the
+    small high-wavenumber lines in FP16's normal range."""
+    import torch
+    dims = (256, 256, 256)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=dims)
+    kx = np.fft.fftfreq(dims[0]) * dims[0]
+    kz = np.arange(dims[2] // 2 + 1)
+    K = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    amp = np.where(K > 0, np.power(np.maximum(K, 1.0), -5.0 / 3.0), 0.0)
+    phase = np.exp(2j * np.pi * rng.uniform(size=amp.shape))
+    x = x + np.fft.irfftn(amp * phase * 1e4, s=dims)
+    x32 = x.astype(np.float32)
+    f = _ffts(dims)
+    X = f.complex_array()
+    f.forward(torch.from_numpy(x32).cuda(), X)
+    Xo = oracle.ffts3d_forward(x32.astype(np.float64))
+    l2, mx = oracle.rel_errors(from_gpu(X), Xo)
+    print(f"[h16] k^-5/3 field {dims}: rel_l2={l2:.3e} rel_max={mx:.3e}", flush=True)
+    assert l2 < TOL_FWD_L2
+    f.close()
+
+
+def test_h16_variants_bitwise_and_autotune(gpu):
+    import torch
+    import fftm_amd
+    from fftm_amd import _lib
+    dims = (256, 256, 256)
+    f = _ffts(dims)
+    L = _lib.lib()
+    u = torch.rand(f.real_shape(), dtype=torch.float32, device="cuda") * 2 - 1
+    X0, y0 = f.complex_array(), f.real_array()
+    f.forward(u, X0)
+    f.backward(X0, y0)
+    nf, nb = f.num_passes()
+    for var in (0, 6, 7):
+        for bw, n in ((0, nf), (1, nb)):
+            for i in range(n):
+                _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
+        X, y = f.complex_array(), f.real_array()
+        f.forward(u, X)
+        f.backward(X, y)
+        assert torch.equal(X, X0) and torch.equal(y, y0), f"variant {var}"
+    assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 1, 3) == 16
+    chosen = f.autotune(warmup=1, iterations=3)
+    assert set(chosen.values()) <= {0, 6, 7}
+    X, y = f.complex_array(), f.real_array()
+    f.forward(u, X)
+    f.backward(X, y)
+    assert torch.equal(X, X0) and torch.equal(y, y0)
+    f.close()

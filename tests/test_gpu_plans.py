@@ -232,3 +232,42 @@ def test_plan_errors_on_gpu(gpu):
     q = fftm_amd.fft(torch.float32, D.C2R, [8, 8], [8, 5], 1, 40, [8, 8], 1, 64, 1, flags=1)
     assert q.get_work_size() == 0
     q.exec(a, b)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n", [32, 64, 512])
+@pytest.mark.parametrize("layout", ["odd_dist", "offset_base"])
+def test_plan_paired_access_fallbacks(gpu, dt, n, layout):
+    """Contiguous lines move two consecutive elements per load / store when every line
+    start is even and the base is 16-byte aligned; odd line pitches and an offset base
+    take the element-wise path.  Both give the same transform."""
+    import torch
+    import fftm_amd
+    D = fftm_amd.direction
+    rt, ct = torch_dtypes(dt)
+    cdt = np.complex64 if dt == np.float32 else np.complex128
+    nh, batch = n // 2 + 1, 5
+    print(f"case paired-access dt={np.dtype(dt).name} n={n} layout={layout}", flush=True)
+    pitch = n + 1 if layout == "odd_dist" else n
+    off = 1 if layout == "offset_base" else 0
+    x = _rng_real((batch, pitch), n + 3).astype(dt)
+    buf = to_gpu(np.concatenate([np.zeros(off, dt), x.ravel()]))
+    p = _plan(dt, D.R2C, [n], [pitch], 1, pitch, [nh], 1, nh, batch)
+    X = _run(p, buf[off:], (batch, nh), ct)
+    check_close(from_gpu(X), np.fft.rfft(x[:, :n].astype(np.float64), axis=1), dt, f"R2C {layout} n={n}")
+    # C2R into an odd-pitch / offset real output
+    yb = torch.zeros(off + batch * pitch, dtype=rt, device="cuda")
+    q = _plan(dt, D.C2R, [n], [nh], 1, nh, [pitch], 1, pitch, batch)
+    work = torch.empty(max(q.get_work_size(), 1), dtype=torch.uint8, device="cuda")
+    q.set_work_area(work)
+    q.exec(X, yb[off:])
+    y = from_gpu(yb[off:]).reshape(batch, pitch)[:, :n]
+    check_close(y, np.fft.irfft(from_gpu(X).astype(np.complex128), n=n, axis=1) * n, dt, f"C2R {layout} n={n}")
+    # C2C along contiguous lines with the same pitch / offset
+    c = _rng_cplx((batch, pitch), n + 4).astype(cdt)
+    cb = to_gpu(np.concatenate([np.zeros(off, cdt), c.ravel()]))
+    ob = torch.zeros(off + batch * pitch, dtype=ct, device="cuda")
+    r = _plan(dt, D.C2CF, [n], [pitch], 1, pitch, [pitch], 1, pitch, batch)
+    r.exec(cb[off:], ob[off:])
+    o = from_gpu(ob[off:]).reshape(batch, pitch)[:, :n]
+    check_close(o, np.fft.fft(c[:, :n].astype(np.complex128), axis=1), dt, f"C2C {layout} n={n}")

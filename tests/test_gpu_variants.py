@@ -30,7 +30,7 @@ def test_variants_bitwise_equal(gpu, dt, kind, dims):
     f.forward(u, X0)
     f.backward(X0, y0)
     nf, nb = f.num_passes()
-    for var in range(1, 6):
+    for var in range(1, 8):
         for bw, n in ((0, nf), (1, nb)):
             for i in range(n):
                 _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
