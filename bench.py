@@ -61,7 +61,11 @@ def _traffic_summary(kernel_key):
         return None
 
 
-def run_gpu(dims, strategy, steps, warmup, check, tune=True):
+def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None):
+    """Time the forward + backward pair.  check: True (FP64 CPU oracle on the same field),
+    "gpu64" (this library's FP64 transform of the same field as the reference, for sizes
+    where the CPU oracle takes too long) or False.  u_dev: a device input (else the
+    splitmix64 field)."""
     import torch
     import fftm_amd
     from fftm_amd import _lib
@@ -69,7 +73,8 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True):
 
     dt = np.float32
     f = fftm_amd.ffts("float32", strategy_4d=strategy)
-    opts = fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", execution="fused", sync=False)
+    opts = fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", execution="fused", sync=False,
+                                      spectrum_storage=storage)
     f.init(*dims, options=opts)
     stream = torch.cuda.current_stream()
     f.set_stream(stream)
@@ -81,12 +86,16 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True):
 
     # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor), resident in HBM
     import oracle
-    if len(dims) == 3:
-        x_host = oracle.fill_random_real_3d(*dims, SEED, dt)
+    if u_dev is not None:
+        u = u_dev
+        x_host = None
     else:
-        x_host = oracle.to_memory(oracle.fill_random_real_4d_logical(*dims, SEED, dt),
-                                  oracle.A0123 if strategy.family.value == "pencil_pencil" else oracle.A3210)
-    u = torch.from_numpy(x_host).cuda()
+        if len(dims) == 3:
+            x_host = oracle.fill_random_real_3d(*dims, SEED, dt)
+        else:
+            x_host = oracle.to_memory(oracle.fill_random_real_4d_logical(*dims, SEED, dt),
+                                      oracle.A0123 if strategy.family.value == "pencil_pencil" else oracle.A3210)
+        u = torch.from_numpy(x_host).cuda()
     X = f.complex_array()
     y = f.real_array()
     nf, nb = f.num_passes()
@@ -130,13 +139,36 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True):
 
     res = {"pair_s": pair_s, "api_pair_s": api_pair_s, "per_pass_ms": per_pass_ms, "pass_bytes": pbytes,
            "passes": passes, "variants": {f"{'bwd' if b else 'fwd'}{i}": v for (b, i), v in variants.items()}}
-    if check:
+    if check == "gpu64":
         pair()
         torch.cuda.synchronize()
+        ud = u.double()
+        yr = y.double() / float(np.prod(dims))
+        res["roundtrip_rms"] = float(torch.sqrt(torch.mean((yr - ud) ** 2)))
+        res["roundtrip_rel_rms"] = float(torch.linalg.vector_norm(yr - ud) / torch.linalg.vector_norm(ud))
+        del yr
+        f.forward(u, X)
+        g = fftm_amd.ffts("float64")
+        g.init(*dims, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key=""))
+        X64 = g.complex_array()
+        g.forward(ud, X64)
+        del ud
+        dX = X.to(torch.complex128) - X64
+        res["fwd_rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(dX) / torch.linalg.vector_norm(X64))
+        res["fwd_rel_max_vs_fp64"] = float(dX.abs().max() / X64.abs().max())
+        res["reference"] = "this library's FP64 transform of the same field on the GPU"
+        del dX, X64
+        g.close()
+    elif check:
+        pair()
+        torch.cuda.synchronize()
+        if x_host is None:
+            x_host = u.cpu().numpy()
         yr = y.cpu().numpy().astype(np.float64) / float(np.prod(dims))
         d = yr - x_host.astype(np.float64)
         res["roundtrip_rms"] = float(np.sqrt(np.mean(d * d)))
         res["roundtrip_max"] = float(np.max(np.abs(d)))
+        res["roundtrip_rel_rms"] = float(np.linalg.norm(d) / np.linalg.norm(x_host.astype(np.float64)))
         f.forward(u, X)
         torch.cuda.synchronize()
         Xg = X.cpu().numpy()
@@ -148,10 +180,104 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True):
         res["fwd_rel_l2_vs_fp64"] = rms
         res["fwd_rel_max_vs_fp64"] = mx
         del Xo, Xg
+    res["bytes_per_pair"] = int(sum(pbytes.values()))
     f.close()
     del u, X, y
     torch.cuda.empty_cache()
     return res
+
+
+def kolmogorov_field(n, seed=5):
+    """uniform(-1,1) plus a k^(-5/3) random-phase field of similar RMS, FP32 on the GPU
+    (synthetic input of the mixed-precision workload)."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    u = torch.rand((n, n, n), generator=g, device="cuda", dtype=torch.float32) * 2 - 1
+    k = torch.fft.fftfreq(n, d=1.0 / n, device="cuda")
+    kz = torch.arange(n // 2 + 1, device="cuda", dtype=torch.float32)
+    K = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    amp = torch.where(K > 0, K.clamp(min=1.0) ** (-5.0 / 3.0), torch.zeros_like(K))
+    del K
+    ph = torch.rand(amp.shape, generator=g, device="cuda") * (2 * math.pi)
+    spec = torch.polar(amp, ph)
+    del amp, ph
+    v = torch.fft.irfftn(spec, s=(n, n, n))
+    del spec
+    v *= (0.5 / float(v.std()))
+    u += v
+    del v
+    torch.cuda.empty_cache()
+    return u
+
+
+def run_poisson(n, steps, warmup, seed=11):
+    """Periodic -Lap u = f on [0, 2 pi)^3 at n^3 FP32: f = 64 random Fourier modes,
+    |k| <= 32, N(0,1) amplitudes, random phases, no zero mode; exact u known mode by mode.
+    The fused filter (5 HBM passes) against forward + spectral multiply + backward."""
+    import torch
+    import fftm_amd
+    rng = np.random.default_rng(seed)
+    modes = []
+    while len(modes) < 64:
+        kv = rng.integers(-32, 33, size=3)
+        if 0 < kv @ kv <= 32 * 32:
+            modes.append(kv)
+    amps = rng.standard_normal(64)
+    phases = rng.uniform(0, 2 * math.pi, 64)
+    xs = torch.arange(n, device="cuda", dtype=torch.float64) * (2 * math.pi / n)
+    rhs = torch.zeros((n, n, n), device="cuda", dtype=torch.float64)
+    exact = torch.zeros_like(rhs)
+    for kv, a, ph in zip(modes, amps, phases):
+        arg = (kv[0] * xs[:, None, None] + kv[1] * xs[None, :, None]) + (kv[2] * xs[None, None, :] + ph)
+        c = torch.cos(arg)
+        rhs += float(a) * c
+        exact += float(a) / float(kv @ kv) * c
+        del arg, c
+    f32 = rhs.float()
+    del rhs
+    f = fftm_amd.ffts("float32")
+    f.init(n, n, n, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", sync=False))
+    stream = torch.cuda.current_stream()
+    f.set_stream(stream)
+    f.autotune(warmup=2, iterations=5)
+    out = torch.empty_like(f32)
+    X = f.complex_array()
+    # the operator is the reference functor's H = -scale/|k|^2 (it solves Lap u = f);
+    # -Lap u = f needs u_hat = f_hat/|k|^2, i.e. scale = -1/N (1/N normalises the pair)
+    scale = -1.0 / float(n) ** 3
+
+    def fused():
+        f.filter(f32, out, "poisson", scale=scale)
+
+    def unfused():
+        f.forward(f32, X)
+        f.spectral_apply(X, X, "poisson", scale=scale)
+        f.backward(X, out)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_unf = timed(unfused)
+    t_fus = timed(fused)
+    fused()
+    torch.cuda.synchronize()
+    err = float((out.double() - exact).abs().max())
+    umax = float(exact.abs().max())
+    f.close()
+    del f32, out, X, exact
+    torch.cuda.empty_cache()
+    return {"workload": f"periodic Poisson solve {n}^3 FP32: R2C, x(-1/|k|^2), C2R, x(1/N); f = 64 random modes "
+                        f"|k|<=32, N(0,1) amplitudes",
+            "ms_fused_filter": round(t_fus * 1e3, 4), "ms_forward_multiply_backward": round(t_unf * 1e3, 4),
+            "hbm_passes_fused": 5, "hbm_passes_unfused": 7,
+            "max_abs_err_vs_exact": err, "rel_max_err_vs_exact": err / umax}
 
 
 def run_cpu_baseline(dims, pairs=2, warmup=1):
@@ -185,6 +311,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-4d", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes (skip plan-time tuning)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 256^3, FP16-storage and Poisson secondary measurements")
+    ap.add_argument("--big", action="store_true", help="also run the 1024^3 FP32 vs FP16-storage comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -271,7 +400,51 @@ def main():
         if not args.no_check:
             out["secondary_64^4"]["errors_vs_fp64_oracle"] = {
                 k: r4[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64", "roundtrip_rms", "roundtrip_max")}
-    out["mixed_precision"] = None
+    if not args.no_extra:
+        r256 = run_gpu((256, 256, 256), fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, False,
+                       not args.no_tune)
+        out["secondary_256^3"] = {"gflops": round(_flops_pair(256 ** 3) / r256["pair_s"] / 1e9, 2),
+                                  "ms_per_pair": round(r256["pair_s"] * 1e3, 4),
+                                  "hbm_gbps_alg": round(_b_alg((256,) * 3, 4) / r256["pair_s"] / 1e9, 1)}
+        # FP16-stored spectra against the plain FP32 pair: same field, same harness
+        rm = run_gpu(dims, None, args.steps, args.warmup, not args.no_check, not args.no_tune, storage="fp16")
+        mp = {"workload": f"ffts 3D pair {n}^3, FP32 arithmetic, FP16 half spectra between passes "
+                          "(per-line power-of-two exponents)",
+              "ms_per_pair": round(rm["pair_s"] * 1e3, 4),
+              "gflops": round(gflops * r3["pair_s"] / rm["pair_s"], 2),
+              "speedup_vs_fp32": round(r3["pair_s"] / rm["pair_s"], 3),
+              "bytes_per_pair": rm["bytes_per_pair"], "bytes_per_pair_fp32": r3["bytes_per_pair"],
+              "hbm_gbps": round(rm["bytes_per_pair"] / rm["pair_s"] / 1e9, 1),
+              "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in rm["per_pass_ms"].items()},
+              "pass_variants": rm["variants"]}
+        for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64", "roundtrip_rel_rms"):
+            if k in rm:
+                mp[k] = rm[k]
+        if "fwd_rel_l2_vs_fp64" in rm:
+            mp["meets_1e-5_rms"] = bool(rm["fwd_rel_l2_vs_fp64"] <= 1e-5)
+        out["mixed_precision"] = mp
+        if args.big:
+            # 1024^3: uniform(-1,1) + k^-5/3 field; FP64 reference from this library on the GPU
+            import torch
+            ub = kolmogorov_field(1024)
+            big = {}
+            for storage in ("fp32", "fp16"):
+                rb = run_gpu((1024,) * 3, None, max(3, args.steps // 4), 2, "gpu64", not args.no_tune,
+                             storage=storage, u_dev=ub)
+                big[storage] = {"ms_per_pair": round(rb["pair_s"] * 1e3, 4),
+                                "gflops": round(_flops_pair(1024 ** 3) / rb["pair_s"] / 1e9, 2),
+                                "hbm_gbps": round(rb["bytes_per_pair"] / rb["pair_s"] / 1e9, 1),
+                                **{k: rb[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64",
+                                                      "roundtrip_rel_rms") if k in rb}}
+            big["speedup_fp16_vs_fp32"] = round(big["fp32"]["ms_per_pair"] / big["fp16"]["ms_per_pair"], 3)
+            big["input"] = "uniform(-1,1) + k^(-5/3) random-phase field (half the RMS), 1024^3 FP32"
+            big["reference"] = "this library's FP64 transform of the same field on the GPU"
+            out["mixed_precision"]["1024^3"] = big
+            del ub
+            torch.cuda.empty_cache()
+        out["poisson_512^3"] = run_poisson(512, args.steps, args.warmup)
+    else:
+        out["mixed_precision"] = None
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(dims)
     print(json.dumps(out), flush=True)

@@ -145,7 +145,10 @@ __device__ __forceinline__ T filter_h( const filter_params<T> &f, float k2 )
         else
             return (T)( f.scale * exp( -(double)k2 * f.inv_two_sigma2 ) );
     }
-    int s = (int)( sqrtf( k2 ) * (float)f.kscale );
+    // shell boundaries fall on exact products (e.g. |k|^2 = 12 m^2 on a cube): evaluate
+    // floor(sqrt(k2) * S / kmax) in FP64 with the same operation order as the radial
+    // multiply, so every bin lands in the same shell on every path
+    int s = (int)( sqrt( (double)k2 ) * f.kscale );
     s     = s < f.nshells - 1 ? s : f.nshells - 1;
     return (T)f.scale * f.coeffs[s];
 }

@@ -91,6 +91,29 @@ struct ffts_init_options
 {
     bool sync                 = true;  // device-synchronise each call (reference safe-call behaviour)
     bool destructive_backward = false; // 4D/3D backward may use its input as scratch (no work area)
+    bool fp16_spectrum_storage = false; // 3D float: FP16 half spectra between passes (FFTM_MI_F32_H16)
+};
+
+// Diagonal spectral operators of ffts::filter / ffts::spectral_apply (fftm_mi_spectral_op).
+template <class T>
+struct spectral_op
+{
+    int      kind    = FFTM_MI_OP_POISSON; // FFTM_MI_OP_POISSON / _GAUSSIAN / _SHELL
+    double   scale   = 1.0;                // e.g. 1/N to normalise a round trip
+    double   sigma   = 0.0;                // Gaussian width (wavenumber units)
+    const T *coeffs  = nullptr;            // shell coefficients (device)
+    int      nshells = 0;
+
+    fftm_mi_spectral_op c() const
+    {
+        fftm_mi_spectral_op o;
+        o.kind    = kind;
+        o.nshells = nshells;
+        o.scale   = scale;
+        o.sigma   = sigma;
+        o.coeffs  = coeffs;
+        return o;
+    }
 };
 
 template <class T, class Strategy4D = strategy_4d_pencil_pencil>
@@ -217,6 +240,29 @@ public:
         run_( false, in.raw_ptr(), out.raw_ptr() );
     }
 
+    // real -> real backward(H forward(in)) with the spectrum kept in the work area; the last
+    // forward axis, the multiply and the first backward axis run as one pass
+    template <std::size_t Dim>
+    void filter( const real_array_t<Dim> &in, real_array_t<Dim> &out, const spectral_op<T> &op )
+    {
+        ensure_dimension_( Dim );
+        if ( h_ == nullptr )
+            throw std::logic_error( "ffts::filter: 3D/4D transforms only" );
+        const fftm_mi_spectral_op o = op.c();
+        ::fftm::wrap::mi::safe_call( fftm_mi_ffts_filter( h_, in.raw_ptr(), out.raw_ptr(), &o, opt_.sync ? 1 : 0 ),
+                                     "fftm_mi_ffts_filter", __FILE__, __LINE__, opt_.sync );
+    }
+    template <std::size_t Dim>
+    void spectral_apply( const complex_array_t<Dim> &in, complex_array_t<Dim> &out, const spectral_op<T> &op )
+    {
+        ensure_dimension_( Dim );
+        if ( h_ == nullptr )
+            throw std::logic_error( "ffts::spectral_apply: 3D/4D transforms only" );
+        const fftm_mi_spectral_op o = op.c();
+        ::fftm::wrap::mi::safe_call( fftm_mi_ffts_spectral_apply( h_, in.raw_ptr(), out.raw_ptr(), &o, opt_.sync ? 1 : 0 ),
+                                     "fftm_mi_ffts_spectral_apply", __FILE__, __LINE__, opt_.sync );
+    }
+
     // memory shapes of the arrays (slowest..fastest)
     std::array<std::size_t, 3> real_shape_3d() const { return { nx_, ny_, nz_ }; }
     std::array<std::size_t, 3> complex_shape_3d() const { return { nx_, ny_, nz_ / 2 + 1 }; }
@@ -257,7 +303,14 @@ private:
     void create_( int layout, const int64_t *n )
     {
         const int flags = opt_.destructive_backward ? FFTM_MI_FLAG_OVERWRITE_INPUT : 0;
-        FFTM_MI_SAFE_CALL( fftm_mi_ffts_create( layout, n, ::fftm::wrap::mi::precision_of<T>::value, flags, &h_ ) );
+        int       prec  = ::fftm::wrap::mi::precision_of<T>::value;
+        if ( opt_.fp16_spectrum_storage )
+        {
+            if ( prec != FFTM_MI_F32 )
+                throw std::logic_error( "ffts: FP16 spectrum storage needs float transforms" );
+            prec = FFTM_MI_F32_H16;
+        }
+        FFTM_MI_SAFE_CALL( fftm_mi_ffts_create( layout, n, prec, flags, &h_ ) );
         std::size_t ws = 0;
         FFTM_MI_SAFE_CALL( fftm_mi_ffts_get_work_size( h_, &ws ) );
         if ( ws > 0 )

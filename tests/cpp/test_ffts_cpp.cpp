@@ -233,6 +233,48 @@ void roundtrip( std::size_t nx, std::size_t ny, std::size_t nz, std::size_t nw, 
     (void)hipEventDestroy( e1 );
 }
 
+// periodic Poisson solve -Lap u = 3 sin x sin y sin z on [0, 2 pi)^3 through ffts::filter
+// (versioned_ffts_3d_test_driver.h:187-209, 640-655: forward, solve functor, backward and
+// the 1/N scale in one call); exact solution sin x sin y sin z
+__global__ void fill_poisson_rhs( float *a, int n, float h )
+{
+    const long long total = (long long)n * n * n;
+    for ( long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+          i += (long long)gridDim.x * blockDim.x )
+    {
+        const long long z = i % n, y = ( i / n ) % n, x = i / ( (long long)n * n );
+        a[i] = -3.0f * sinf( h * x ) * sinf( h * y ) * sinf( h * z );
+    }
+}
+
+void poisson_filter()
+{
+    const int                 n = 64;
+    fftm::mi::ffts<float>     f;
+    f.init( n, n, n );
+    fftm::mi::device_array<float, 3> rhs, u;
+    rhs.init( n, n, n );
+    u.init( n, n, n );
+    const double h = 2.0 * 3.14159265358979323846 / n;
+    hipLaunchKernelGGL( fill_poisson_rhs, dim3( 256 ), dim3( 256 ), 0, 0, rhs.raw_ptr(), n, (float)h );
+    fftm::mi::spectral_op<float> op;
+    op.kind  = FFTM_MI_OP_POISSON;
+    op.scale = 1.0 / ( (double)n * n * n );
+    f.filter( rhs, u, op );
+    std::vector<float> hu( (std::size_t)n * n * n );
+    fftm::wrap::mi::mi_runtime_api::memcpy( hu.data(), u.raw_ptr(), hu.size() * sizeof( float ) );
+    double err = 0;
+    for ( int x = 0; x < n; ++x )
+        for ( int y = 0; y < n; ++y )
+            for ( int z = 0; z < n; ++z )
+            {
+                const double ex = std::sin( h * x ) * std::sin( h * y ) * std::sin( h * z );
+                err             = std::max( err, std::fabs( hu[( (std::size_t)x * n + y ) * n + z] - ex ) );
+            }
+    std::printf( "poisson filter 64^3 f32: max |u - exact| = %.3e\n", err );
+    expect( err < 2e-6, "poisson filter" );
+}
+
 } // namespace
 
 int main()
@@ -240,6 +282,7 @@ int main()
     try
     {
         plan_smoke();
+        poisson_filter();
         using P = fftm::mi::strategy_4d_pencil_pencil;
         using S = fftm::mi::strategy_4d_slab_slab;
         using X = fftm::mi::strategy_4d_slab_native_xzwy;
