@@ -261,11 +261,12 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 //   5: twice the columns and twice the threads per column (1024-thread workgroups)
 //   6: the tile of 0, exchanged through LDS in two rounds (real parts, then imaginary
 //      parts): half the LDS per workgroup, so more tiles in flight per CU
-//   7: the tile of 4, exchanged in two rounds
+//   7: the tile of 0 exchanged in two rounds, registers capped for 4 waves per SIMD
+//      (some spilling; wins on the longest contiguous R2C lines)
 constexpr int NUM_VARIANTS = 8;
 
 constexpr bool variant_split( int var ) { return var >= 6; }
-constexpr int  variant_shape( int var ) { return var == 6 ? 0 : ( var == 7 ? 4 : var ); }
+constexpr int  variant_shape( int var ) { return var >= 6 ? 0 : var; }
 
 constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 {
@@ -503,10 +504,23 @@ struct pass_kernel
     static constexpr int   PB  = TW;
     static constexpr int   LDS = LOADB ? N * PB : TW * PA;
     static constexpr int   CPT = KIND == KIND_C2C ? 1 : 2;
-    static_assert( LDS * (int)sizeof( C ) <= ( variant_shape( VAR ) >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
+    static_assert( LDS * (int)sizeof( C ) / ( SPLIT ? 2 : 1 ) <=
+                       ( variant_shape( VAR ) >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
                    "LDS tile too large" );
     // LDS words: complex values, or one component at a time with the split exchange
     using LW = typename std::conditional<SPLIT, T, C>::type;
+    // C2R along contiguous lines stages its half spectra in LDS (whole-value words)
+    static constexpr bool STAGE_C2R = KIND == KIND_C2R && !LOADB && !SPLIT && 2 * TW * ( N / 2 + 1 ) <= LDS;
+    // the split exchange halves the LDS tile so that more workgroups share a CU: ask the
+    // register allocator for the matching occupancy (waves per SIMD)
+    static constexpr int WAVES_PER_EU = [] {
+        if ( VAR != 7 )
+            return 1;
+        const int lds_wg  = LDS * (int)sizeof( LW );
+        const int wgs     = lds_wg > 0 ? 163840 / lds_wg : 8;
+        const int waves   = wgs * ( NT / 64 ) / 4;
+        return waves < 1 ? 1 : ( waves > 4 ? 4 : waves );
+    }();
 
     __device__ static __forceinline__ int L( int n, int c )
     {
@@ -922,18 +936,23 @@ struct pass_kernel
             constexpr int NB2  = ( NBF2 + TPC - 1 ) / TPC;
             constexpr int TST  = N / ( NS2 * R2 );
             // twiddles of the next stage are fetched before the exchange, so their
-            // latency hides behind the LDS writes and the barrier
-            C tws[NB2][R2];
-#pragma unroll
-            for ( int b = 0; b < NB2; ++b )
+            // latency hides behind the LDS writes and the barrier (the split-exchange
+            // variants keep registers for occupancy instead and fetch them after it)
+            constexpr bool TWPF = !SPLIT;
+            C              tws[NB2][R2];
+            if constexpr ( TWPF )
             {
-                const int j = jmap<S + 1>( x.t, b );
-                if ( NBF2 % TPC == 0 || j < NBF2 )
-                {
-                    const int k = j % NS2;
 #pragma unroll
-                    for ( int r = 1; r < R2; ++r )
-                        tws[b][r] = x.p->tw[r * k * TST];
+                for ( int b = 0; b < NB2; ++b )
+                {
+                    const int j = jmap<S + 1>( x.t, b );
+                    if ( NBF2 % TPC == 0 || j < NBF2 )
+                    {
+                        const int k = j % NS2;
+#pragma unroll
+                        for ( int r = 1; r < R2; ++r )
+                            tws[b][r] = x.p->tw[r * k * TST];
+                    }
                 }
             }
             C w[NB2][R2];
@@ -959,6 +978,13 @@ struct pass_kernel
                 const int j = jmap<S + 1>( x.t, b );
                 if ( NBF2 % TPC == 0 || j < NBF2 )
                 {
+                    if constexpr ( !TWPF )
+                    {
+                        const int k = j % NS2;
+#pragma unroll
+                        for ( int r = 1; r < R2; ++r )
+                            tws[b][r] = x.p->tw[r * k * TST];
+                    }
 #pragma unroll
                     for ( int r = 1; r < R2; ++r )
                         w[b][r] = twmul<INV>( w[b][r], tws[b][r] );
@@ -1021,6 +1047,92 @@ struct pass_kernel
         constexpr int NB0  = ( NBF0 + TPC - 1 ) / TPC;
         C             v[NB0][R0];
         bool          loaded = false;
+        if constexpr ( STAGE_C2R )
+        {
+            // C2R along contiguous lines: the tile's 2 TW half spectra go through LDS with
+            // coalesced loads, each bin read once (the Hermitian extension reads every
+            // interior bin twice); stage 0 then builds z = A~ + i B~ from LDS
+            constexpr int NH = N / 2 + 1;
+            __shared__ int64_t loff[2 * TW], xoff[2 * TW];
+            C                 *st = reinterpret_cast<C *>( lds );
+            // 2 TW lines can exceed the thread count (TPC = 1 for N <= 16): strided fill
+            for ( int l = tid; l < 2 * TW; l += NT )
+            {
+                const int64_t c  = x.tile * TW * 2 + l;
+                int64_t       io = 0, oo = 0, xi = 0, yo = 0;
+                col_offsets( p, c < p.ncols ? c : 0, io, oo );
+                if constexpr ( IOM & IO_H16_IN )
+                    col_exponents( p, c < p.ncols ? c : 0, xi, yo );
+                loff[l] = c < p.ncols ? io : -1;
+                xoff[l] = xi;
+            }
+            __syncthreads();
+            // every load is issued before the first LDS write
+            constexpr int ITEMS = 2 * TW * NH;
+            constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
+            C             zs[PER];
+#pragma unroll
+            for ( int i = 0; i < PER; ++i )
+            {
+                const int e = tid + i * NT;
+                zs[i]       = mk<T>( 0, 0 );
+                if ( ITEMS % NT == 0 || e < ITEMS )
+                {
+                    const int     line = e / NH, k = e % NH;
+                    const int64_t lo   = loff[line];
+                    if ( lo >= 0 )
+                    {
+                        if constexpr ( IOM & IO_H16_IN )
+                            zs[i] = h16_load( p.in, lo + (int64_t)k * p.ies, p.xin, xoff[line] + (int64_t)k * p.xsn );
+                        else
+                            zs[i] = static_cast<const C *>( p.in )[lo + (int64_t)k * p.ies];
+                    }
+                }
+            }
+#pragma unroll
+            for ( int i = 0; i < PER; ++i )
+            {
+                const int e = tid + i * NT;
+                if ( ITEMS % NT == 0 || e < ITEMS )
+                    st[e] = zs[i]; // line * NH + k == e
+            }
+            __syncthreads();
+#pragma unroll
+            for ( int b = 0; b < NB0; ++b )
+            {
+                const int j = jmap<0>( x.t, b );
+                if ( NBF0 % TPC == 0 || j < NBF0 )
+                {
+#pragma unroll
+                    for ( int r = 0; r < R0; ++r )
+                    {
+                        const int n  = j + r * NBF0;
+                        const int nn = n <= N / 2 ? n : N - n;
+                        C         a  = st[( 2 * x.c ) * NH + nn];
+                        C         bq = st[( 2 * x.c + 1 ) * NH + nn];
+                        if ( n > N / 2 )
+                        {
+                            a.y  = -a.y;
+                            bq.y = -bq.y;
+                        }
+                        if ( n == 0 || 2 * n == N )
+                        {
+                            a.y  = 0;
+                            bq.y = 0;
+                        }
+                        else if ( p.wmode == 2 )
+                        {
+                            a  = scl( a, (T)0.5 );
+                            bq = scl( bq, (T)0.5 );
+                        }
+                        v[b][r] = mk<T>( a.x - bq.y, a.y + bq.x );
+                    }
+                    dft_fixed<R0, INV>( v[b], p.tw, N );
+                }
+            }
+            __syncthreads(); // the exchange buffer aliases the staging area
+            loaded = true;
+        }
         if constexpr ( PAIR_LOAD )
         {
             if ( p.vec & 1 )
@@ -1088,6 +1200,7 @@ struct pass_kernel
 
 template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32>
 __global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::NT ) )
+    __attribute__( ( amdgpu_waves_per_eu( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::WAVES_PER_EU ) ) )
     k_pass( const pass_params<T> p )
 {
     pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::run( p );

@@ -176,9 +176,10 @@ int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, cons
 /* Tile-shape variant (0..7) of one pass's kernel: 0 default, 1 half the columns per
  * workgroup, 2 twice the threads per column, 3 both, 4 twice the columns (256-byte row
  * segments, up to 160 KB of LDS), 5 twice the columns and the threads per column,
- * 6 and 7 the tiles of 0 and 4 exchanged through LDS one component at a time (half
- * the LDS per workgroup).  Results are bitwise identical across variants; only the
- * speed differs.  Set by the tuner or by hand. */
+ * 6 the tile of 0 exchanged through LDS one component at a time (half the LDS per
+ * workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Results are bitwise
+ * identical across variants; only the speed differs.  Set by the tuner or by hand.
+ * FP16-storage passes (FFTM_MI_F32_H16) have variants 0, 6 and 7. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
 int fftm_mi_ffts_get_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t *variant );
 /* Algorithmic HBM bytes one pass moves (reads + writes of its input and output). */
