@@ -347,14 +347,14 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
             return launch_pass_mixed_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, r.iom, s );
         }
         if ( r.filter )
-            return launch_filter_f32( p, *static_cast<const filter_params<float> *>( filter ), (int)r.N, r.loadb, s );
+            return launch_filter_f32( p, *static_cast<const filter_params<float> *>( filter ), (int)r.N, r.loadb, r.var, s );
         return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
                          : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
     }
     else
     {
         if ( r.filter )
-            return launch_filter_f64( p, *static_cast<const filter_params<double> *>( filter ), (int)r.N, r.loadb, s );
+            return launch_filter_f64( p, *static_cast<const filter_params<double> *>( filter ), (int)r.N, r.loadb, r.var, s );
         return r.generic ? launch_generic_f64( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
                          : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
     }
@@ -1594,8 +1594,9 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
                          lf.src == BUF_WORK && lf.dst == BUF_WORK && fb.dst == BUF_WORK;
     if ( fusable )
     {
+        // the tile shape follows the tuned forward pass it replaces (shapes 0, 2, 4)
         pre.back().filter = true;
-        pre.back().var    = 0;
+        pre.back().var    = ( lf.var == 2 || lf.var == 4 ) ? lf.var : ( lf.var == 5 ? 4 : ( lf.var == 3 ? 2 : 0 ) );
         post.erase( post.begin() );
     }
     h.flt_pre   = pre;

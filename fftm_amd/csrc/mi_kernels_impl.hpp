@@ -95,21 +95,31 @@ int tile_cols_n( bool loadb, int var )
 }
 
 // fused filter pass (forward C2C, x H(k), inverse C2C), default tile shape
-template <typename T, int N>
-hipError_t launch_filter_n( const pass_params<T> &p, const filter_params<T> &f, bool loadb, hipStream_t s )
+template <typename T, int N, bool LB, int VAR>
+hipError_t launch_filter_nv( const pass_params<T> &p, const filter_params<T> &f, hipStream_t s )
 {
-    auto go = [&]( auto lb ) -> hipError_t {
-        constexpr bool LB    = decltype( lb )::value;
-        using K              = pass_kernel<T, N, KIND_C2C, false, LB, 0, IO_F32, true>;
-        const int64_t ntiles = ( p.ncols + K::TW - 1 ) / K::TW;
-        if ( ntiles <= 0 )
-            return hipSuccess;
-        if ( ntiles > 0x7fffffffLL )
-            return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL( ( k_pass_filter<T, N, LB, 0> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p, f );
-        return hipGetLastError();
-    };
-    return loadb ? go( std::true_type{} ) : go( std::false_type{} );
+    using K              = pass_kernel<T, N, KIND_C2C, false, LB, VAR, IO_F32, true>;
+    const int64_t ntiles = ( p.ncols + K::TW - 1 ) / K::TW;
+    if ( ntiles <= 0 )
+        return hipSuccess;
+    if ( ntiles > 0x7fffffffLL )
+        return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL( ( k_pass_filter<T, N, LB, VAR> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p, f );
+    return hipGetLastError();
+}
+
+// tile shapes of the filter pass: 0, 2 (twice the threads per column), 4 (twice the columns)
+template <typename T, int N>
+hipError_t launch_filter_n( const pass_params<T> &p, const filter_params<T> &f, bool loadb, int var, hipStream_t s )
+{
+    if constexpr ( variants_for<N>() > 1 )
+    {
+        if ( var == 2 )
+            return loadb ? launch_filter_nv<T, N, true, 2>( p, f, s ) : launch_filter_nv<T, N, false, 2>( p, f, s );
+        if ( var == 4 )
+            return loadb ? launch_filter_nv<T, N, true, 4>( p, f, s ) : launch_filter_nv<T, N, false, 4>( p, f, s );
+    }
+    return loadb ? launch_filter_nv<T, N, true, 0>( p, f, s ) : launch_filter_nv<T, N, false, 0>( p, f, s );
 }
 
 // FP16-stored spectra (FP32 arithmetic): the storage modes each kind can have
@@ -225,23 +235,23 @@ hipError_t launch_generic_t( const pass_params<T> &p, const radix_list &rl, int 
 #define FFTM_MI_DEFINE_FILTER( T, TN )                                                                                 \
     namespace fftm_mi                                                                                                  \
     {                                                                                                                  \
-    hipError_t launch_filter_##TN( const pass_params<T> &p, const filter_params<T> &f, int N, bool loadb,              \
+    hipError_t launch_filter_##TN( const pass_params<T> &p, const filter_params<T> &f, int N, bool loadb, int var,     \
                                    hipStream_t s )                                                                     \
     {                                                                                                                  \
         switch ( N )                                                                                                   \
         {                                                                                                              \
-        case 2: return detail::launch_filter_n<T, 2>( p, f, loadb, s );                                                \
-        case 4: return detail::launch_filter_n<T, 4>( p, f, loadb, s );                                                \
-        case 8: return detail::launch_filter_n<T, 8>( p, f, loadb, s );                                                \
-        case 16: return detail::launch_filter_n<T, 16>( p, f, loadb, s );                                              \
-        case 32: return detail::launch_filter_n<T, 32>( p, f, loadb, s );                                              \
-        case 64: return detail::launch_filter_n<T, 64>( p, f, loadb, s );                                              \
-        case 128: return detail::launch_filter_n<T, 128>( p, f, loadb, s );                                            \
-        case 256: return detail::launch_filter_n<T, 256>( p, f, loadb, s );                                            \
-        case 512: return detail::launch_filter_n<T, 512>( p, f, loadb, s );                                            \
-        case 1024: return detail::launch_filter_n<T, 1024>( p, f, loadb, s );                                          \
-        case 2048: return detail::launch_filter_n<T, 2048>( p, f, loadb, s );                                          \
-        case 4096: return detail::launch_filter_n<T, 4096>( p, f, loadb, s );                                          \
+        case 2: return detail::launch_filter_n<T, 2>( p, f, loadb, var, s );                                                \
+        case 4: return detail::launch_filter_n<T, 4>( p, f, loadb, var, s );                                                \
+        case 8: return detail::launch_filter_n<T, 8>( p, f, loadb, var, s );                                                \
+        case 16: return detail::launch_filter_n<T, 16>( p, f, loadb, var, s );                                              \
+        case 32: return detail::launch_filter_n<T, 32>( p, f, loadb, var, s );                                              \
+        case 64: return detail::launch_filter_n<T, 64>( p, f, loadb, var, s );                                              \
+        case 128: return detail::launch_filter_n<T, 128>( p, f, loadb, var, s );                                            \
+        case 256: return detail::launch_filter_n<T, 256>( p, f, loadb, var, s );                                            \
+        case 512: return detail::launch_filter_n<T, 512>( p, f, loadb, var, s );                                            \
+        case 1024: return detail::launch_filter_n<T, 1024>( p, f, loadb, var, s );                                          \
+        case 2048: return detail::launch_filter_n<T, 2048>( p, f, loadb, var, s );                                          \
+        case 4096: return detail::launch_filter_n<T, 4096>( p, f, loadb, var, s );                                          \
         default: return hipErrorInvalidValue;                                                                          \
         }                                                                                                              \
     }                                                                                                                  \

@@ -42,9 +42,9 @@ hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, 
                                   hipStream_t s );
 
 // fused filter pass along a compiled length (forward C2C, x H(k), inverse C2C)
-hipError_t launch_filter_f32( const pass_params<float> &p, const filter_params<float> &f, int N, bool loadb,
+hipError_t launch_filter_f32( const pass_params<float> &p, const filter_params<float> &f, int N, bool loadb, int var,
                               hipStream_t s );
-hipError_t launch_filter_f64( const pass_params<double> &p, const filter_params<double> &f, int N, bool loadb,
+hipError_t launch_filter_f64( const pass_params<double> &p, const filter_params<double> &f, int N, bool loadb, int var,
                               hipStream_t s );
 
 hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
