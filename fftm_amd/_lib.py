@@ -54,13 +54,14 @@ class FftmMiError(RuntimeError):
         super().__init__(f"{where}: {_STATUS_NAMES.get(status, 'unknown')} ({status}): {message}")
 
 
-OP_POISSON, OP_GAUSSIAN, OP_SHELL = 1, 2, 3
+OP_POISSON, OP_GAUSSIAN, OP_SHELL, OP_DERIVATIVE, OP_POISSON_GRADIENT, OP_BANDPASS = 1, 2, 3, 4, 5, 6
 
 
 class SpectralOp(ctypes.Structure):
     """fftm_mi_spectral_op."""
     _fields_ = [("kind", ctypes.c_int32), ("nshells", ctypes.c_int32), ("scale", ctypes.c_double),
-                ("sigma", ctypes.c_double), ("coeffs", ctypes.c_void_p)]
+                ("sigma", ctypes.c_double), ("coeffs", ctypes.c_void_p), ("axis", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("klo", ctypes.c_double), ("khi", ctypes.c_double)]
 
 
 class PlanDesc(ctypes.Structure):

@@ -76,6 +76,7 @@ struct pass_rt
     // strides (input: batch dims and transform axis; output: batch dims) and the byte
     // offsets of the exponent arrays in the work area
     bool       filter   = false; // fused filter pass (forward C2C, x H(k), inverse C2C)
+    int        ax       = -1;    // logical axis transformed (ffts passes)
     int        iom      = 0;
     int64_t    xsn = 0, xs0 = 0, xs1 = 0, xs2 = 0, ys0 = 0, ys1 = 0, ys2 = 0;
     int64_t    xoff = 0, yoff = 0;
@@ -645,6 +646,7 @@ pass_rt axis_pass( int kind, bool inv, int ax, int dim, const int64_t *ext, cons
     p.N    = N;
     p.src  = src;
     p.dst  = dst;
+    p.ax   = ax;
     p.ies  = si[ax];
     p.oes  = so[ax];
     std::vector<bdim> v;
@@ -686,6 +688,7 @@ pass_rt axis_pass_h16( int kind, bool inv, int ax, int dim, const int64_t *ext, 
     p.dst  = dst;
     p.ies  = si[ax];
     p.oes  = so[ax];
+    p.ax   = ax;
     p.iom  = ( xs ? IO_H16_IN : 0 ) | ( ys ? IO_H16_OUT : 0 );
     p.xsn  = xs ? xs[ax] : 0;
     std::vector<bdim> v;
@@ -965,6 +968,7 @@ struct fftm_mi_ffts_s
     // (or the forward passes, an elementwise multiply and the backward passes)
     std::vector<pass_rt> flt_pre, flt_post;
     bool                 flt_ready = false, flt_fused = false;
+    int                  flt_line_ax = -1; // logical axis of the fused filter pass's lines
 
     ~fftm_mi_ffts_s()
     {
@@ -1529,8 +1533,13 @@ int check_op( const fftm_mi_ffts_s &h, const fftm_mi_spectral_op *op, const char
 {
     if ( !op )
         return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": null operator" );
-    if ( op->kind < FFTM_MI_OP_POISSON || op->kind > FFTM_MI_OP_SHELL )
+    if ( op->kind < FFTM_MI_OP_POISSON || op->kind > FFTM_MI_OP_BANDPASS )
         return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": unknown operator kind" );
+    if ( ( op->kind == FFTM_MI_OP_DERIVATIVE || op->kind == FFTM_MI_OP_POISSON_GRADIENT ) &&
+         ( op->axis < 0 || op->axis >= h.sg.nd ) )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": derivative axis out of range" );
+    if ( op->kind == FFTM_MI_OP_BANDPASS && !( op->khi >= op->klo && op->klo >= 0.0 ) )
+        return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": band-pass needs 0 <= klo <= khi" );
     if ( op->kind == FFTM_MI_OP_GAUSSIAN && !( op->sigma > 0.0 ) )
         return fail( FFTM_MI_INVALID_VALUE, std::string( what ) + ": Gaussian sigma must be > 0" );
     if ( op->kind == FFTM_MI_OP_SHELL && ( !op->coeffs || op->nshells < 1 || op->nshells > MAX_SHELLS ) )
@@ -1540,18 +1549,26 @@ int check_op( const fftm_mi_ffts_s &h, const fftm_mi_spectral_op *op, const char
 }
 
 template <typename T>
-filter_params<T> make_filter_params( const fftm_mi_ffts_s &h, const fftm_mi_spectral_op &op )
+filter_params<T> make_filter_params( const fftm_mi_ffts_s &h, const fftm_mi_spectral_op &op, int line_ax = -1 )
 {
     filter_params<T> f{};
-    f.kind    = op.kind;
-    f.nshells = op.kind == FFTM_MI_OP_SHELL ? op.nshells : 1;
-    f.nd      = h.sg.nd;
+    f.kind     = op.kind;
+    f.nshells  = op.kind == FFTM_MI_OP_SHELL ? op.nshells : 1;
+    f.nd       = h.sg.nd;
+    f.dax      = -1;
+    f.line_m = -2;
     for ( int m = 0; m < h.sg.nd; ++m )
     {
         const int a = h.sg.ax[m];
         f.ext[m]    = h.sg.ext[m];
         f.nlen[m]   = a == h.sg.half_axis ? 0 : h.sg.n[a];
+        if ( ( op.kind == FFTM_MI_OP_DERIVATIVE || op.kind == FFTM_MI_OP_POISSON_GRADIENT ) && a == op.axis )
+            f.dax = m;
+        if ( a == line_ax )
+            f.line_m = m;
     }
+    f.klo2 = op.klo * op.klo;
+    f.khi2 = op.khi * op.khi;
     f.scale          = op.scale;
     f.inv_two_sigma2 = op.kind == FFTM_MI_OP_GAUSSIAN ? 1.0 / ( 2.0 * op.sigma * op.sigma ) : 0.0;
     f.kscale         = h.sg.kscale * (double)f.nshells;
@@ -1599,9 +1616,10 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
         pre.back().var    = ( lf.var == 2 || lf.var == 4 ) ? lf.var : ( lf.var == 5 ? 4 : ( lf.var == 3 ? 2 : 0 ) );
         post.erase( post.begin() );
     }
-    h.flt_pre   = pre;
-    h.flt_post  = post;
-    h.flt_fused = fusable;
+    h.flt_pre     = pre;
+    h.flt_post    = post;
+    h.flt_fused   = fusable;
+    h.flt_line_ax = fusable ? lf.ax : -1;
     h.flt_ready = true;
     return ok();
 }
@@ -1622,7 +1640,7 @@ int spectral_op_launch( fftm_mi_ffts_s &h, const filter_params<T> &f, const void
 template <typename T>
 int filter_run( fftm_mi_ffts_s &h, const void *in, void *out, const fftm_mi_spectral_op &op, bool sync )
 {
-    const filter_params<T> f  = make_filter_params<T>( h, op );
+    const filter_params<T> f  = make_filter_params<T>( h, op, h.flt_line_ax );
     int                    rc = run_passes( h.flt_pre, h.tb, h.prec, in, out, h.work, h.stream, false, false,
                                             "ffts_filter", 0, &f );
     if ( rc == FFTM_MI_SUCCESS && !h.flt_fused )

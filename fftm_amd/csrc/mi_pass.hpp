@@ -96,9 +96,12 @@ typedef float f32x4 __attribute__( ( ext_vector_type( 4 ) ) );
 // same lines (the fused filter pass): H(k) from the wavenumber vector of each bin.
 enum : int
 {
-    OP_POISSON  = 1, // H = -scale / |k|^2, H(0) = 0   (periodic -Laplace inverse)
-    OP_GAUSSIAN = 2, // H = scale * exp(-|k|^2 / (2 sigma^2))
-    OP_SHELL    = 3  // H = scale * coeffs[min(S - 1, floor(S |k| / kmax))]
+    OP_POISSON       = 1, // H = -scale / |k|^2, H(0) = 0   (periodic -Laplace inverse)
+    OP_GAUSSIAN      = 2, // H = scale * exp(-|k|^2 / (2 sigma^2))
+    OP_SHELL         = 3, // H = scale * coeffs[min(S - 1, floor(S |k| / kmax))]
+    OP_DERIVATIVE    = 4, // H = i scale k_a                 (d / dx_a)
+    OP_POISSON_GRAD  = 5, // H = -i scale k_a / |k|^2, H(0) = 0 (gradient of the Poisson solve)
+    OP_BANDPASS      = 6  // H = scale where klo <= |k| <= khi, else 0
 };
 
 template <typename T>
@@ -107,21 +110,26 @@ struct filter_params
     int32_t  kind;
     int32_t  nshells;
     int32_t  nd;       // memory axes of the (dense) spectrum array, slowest first
+    int32_t  dax;      // memory axis of the derivative (OP_DERIVATIVE, OP_POISSON_GRAD)
+    int32_t  line_m; // memory axis of the fused filter pass's lines
     int32_t  pad_;
     int64_t  ext[4];   // memory extents
     int64_t  nlen[4];  // full logical length of each memory axis; 0 marks the halved real axis
     double   scale;
     double   inv_two_sigma2;
     double   kscale;   // S / kmax
+    double   klo2, khi2; // OP_BANDPASS: squared band edges
     const T *coeffs;   // OP_SHELL coefficients (T[nshells])
 };
 
-// sum of k_a^2 over the memory axes, for the bin at spectrum offset off
-// (wavenumbers: k = i on the halved axis, k = i <= n/2 ? i : i - n elsewhere)
+// wavenumbers of the bin at spectrum offset off: returns sum_a k_a^2 and sets kd to the
+// wavenumber along memory axis f.dax (k = i on the halved axis, k = i <= n/2 ? i : i - n
+// elsewhere)
 template <typename T>
-__device__ __forceinline__ float filter_k2( const filter_params<T> &f, int64_t off )
+__device__ __forceinline__ float filter_k2( const filter_params<T> &f, int64_t off, float &kd )
 {
     float k2 = 0.f;
+    kd       = 0.f;
     for ( int m = f.nd - 1; m >= 0; --m )
     {
         const int64_t idx = off % f.ext[m];
@@ -129,28 +137,47 @@ __device__ __forceinline__ float filter_k2( const filter_params<T> &f, int64_t o
         const int64_t n = f.nlen[m];
         const int64_t k = ( n == 0 || idx <= n / 2 ) ? idx : idx - n;
         k2 += (float)( k * k );
+        if ( m == f.dax )
+            kd = (float)k;
     }
     return k2;
 }
 
+// H(k) as (real, imaginary) parts
 template <typename T>
-__device__ __forceinline__ T filter_h( const filter_params<T> &f, float k2 )
+__device__ __forceinline__ cpx<T> filter_h( const filter_params<T> &f, float k2, float kd )
 {
-    if ( f.kind == OP_POISSON )
-        return k2 == 0.f ? (T)0 : (T)( -f.scale ) / (T)k2;
-    if ( f.kind == OP_GAUSSIAN )
+    switch ( f.kind )
     {
+    case OP_POISSON: return mk<T>( k2 == 0.f ? (T)0 : (T)( -f.scale ) / (T)k2, (T)0 );
+    case OP_GAUSSIAN:
         if constexpr ( sizeof( T ) == 4 )
-            return (T)f.scale * __expf( -k2 * (float)f.inv_two_sigma2 );
+            return mk<T>( (T)f.scale * __expf( -k2 * (float)f.inv_two_sigma2 ), (T)0 );
         else
-            return (T)( f.scale * exp( -(double)k2 * f.inv_two_sigma2 ) );
+            return mk<T>( (T)( f.scale * exp( -(double)k2 * f.inv_two_sigma2 ) ), (T)0 );
+    case OP_DERIVATIVE: return mk<T>( (T)0, (T)f.scale * (T)kd );
+    case OP_POISSON_GRAD: return mk<T>( (T)0, k2 == 0.f ? (T)0 : (T)( -f.scale ) * (T)kd / (T)k2 );
+    case OP_BANDPASS:
+        return mk<T>( ( (double)k2 >= f.klo2 && (double)k2 <= f.khi2 ) ? (T)f.scale : (T)0, (T)0 );
+    default:
+    {
+        // shell boundaries fall on exact products (e.g. |k|^2 = 12 m^2 on a cube):
+        // evaluate floor(sqrt(k2) * S / kmax) in FP64 with the same operation order as
+        // the radial multiply, so every bin lands in the same shell on every path
+        int s = (int)( sqrt( (double)k2 ) * f.kscale );
+        s     = s < f.nshells - 1 ? s : f.nshells - 1;
+        return mk<T>( (T)f.scale * f.coeffs[s], (T)0 );
     }
-    // shell boundaries fall on exact products (e.g. |k|^2 = 12 m^2 on a cube): evaluate
-    // floor(sqrt(k2) * S / kmax) in FP64 with the same operation order as the radial
-    // multiply, so every bin lands in the same shell on every path
-    int s = (int)( sqrt( (double)k2 ) * f.kscale );
-    s     = s < f.nshells - 1 ? s : f.nshells - 1;
-    return (T)f.scale * f.coeffs[s];
+    }
+}
+
+// z * H with H real or purely imaginary (exact: no rounding from zero parts)
+template <typename T>
+__device__ __forceinline__ cpx<T> apply_h( cpx<T> z, cpx<T> h, int kind )
+{
+    if ( kind == OP_DERIVATIVE || kind == OP_POISSON_GRAD )
+        return mk<T>( -z.y * h.y, z.x * h.y );
+    return mk<T>( z.x * h.x, z.y * h.x );
 }
 
 // per-workgroup state of a pass kernel (shared by the forward and inverse halves of
@@ -496,7 +523,10 @@ struct pass_kernel
     using C                    = cpx<T>;
     static constexpr rplan RP  = make_rplan( N );
     static constexpr bool  SPLIT = variant_split( VAR );
-    static constexpr geom  G   = pick_geom( N, (int)sizeof( C ), LOADB, variant_shape( VAR ) );
+    // FP16-stored lines move 4 bytes per bin: their variant 7 doubles the columns (128-byte
+    // FP16 row segments) with the split exchange, instead of capping registers
+    static constexpr int   SHAPE = ( IOM != IO_F32 && VAR == 7 ) ? 4 : variant_shape( VAR );
+    static constexpr geom  G   = pick_geom( N, (int)sizeof( C ), LOADB, SHAPE );
     static constexpr int   TW  = G.tw;
     static constexpr int   TPC = G.tpc;
     static constexpr int   NT  = TW * TPC;
@@ -505,7 +535,7 @@ struct pass_kernel
     static constexpr int   LDS = LOADB ? N * PB : TW * PA;
     static constexpr int   CPT = KIND == KIND_C2C ? 1 : 2;
     static_assert( LDS * (int)sizeof( C ) / ( SPLIT ? 2 : 1 ) <=
-                       ( variant_shape( VAR ) >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
+                       ( SHAPE >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
                    "LDS tile too large" );
     // LDS words: complex values, or one component at a time with the split exchange
     using LW = typename std::conditional<SPLIT, T, C>::type;
@@ -514,7 +544,7 @@ struct pass_kernel
     // the split exchange halves the LDS tile so that more workgroups share a CU: ask the
     // register allocator for the matching occupancy (waves per SIMD)
     static constexpr int WAVES_PER_EU = [] {
-        if ( VAR != 7 )
+        if ( VAR != 7 || IOM != IO_F32 )
             return 1;
         const int lds_wg  = LDS * (int)sizeof( LW );
         const int wgs     = lds_wg > 0 ? 163840 / lds_wg : 8;
@@ -660,7 +690,11 @@ struct pass_kernel
         {
             // forward transform of the lines done (natural order: element d + m NS):
             // multiply by H(k), then run the inverse transform of the same lines from LDS
-            const float k2c = filter_k2( *x.f, x.oa );
+            // the column's offset gives the wavenumbers off the line axis; the element
+            // index along the line gives the line's own
+            float       kdc = 0.f;
+            const float k2c = filter_k2( *x.f, x.oa, kdc );
+            const bool  dline = x.f->dax == x.f->line_m;
 #pragma unroll
             for ( int b = 0; b < NB; ++b )
             {
@@ -673,7 +707,9 @@ struct pass_kernel
                     {
                         const int n  = d + m * NS;
                         const int kk = n <= N / 2 ? n : n - N;
-                        v[b][m]      = scl( v[b][m], filter_h( *x.f, k2c + (float)( kk * kk ) ) );
+                        v[b][m]      = apply_h( v[b][m],
+                                                filter_h( *x.f, k2c + (float)( kk * kk ), dline ? (float)kk : kdc ),
+                                                x.f->kind );
                     }
                 }
             }

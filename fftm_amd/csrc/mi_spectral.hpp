@@ -63,7 +63,11 @@ __global__ void __launch_bounds__( 256 )
     k_spectral_op( const filter_params<T> f, const cpx<T> *in, cpx<T> *out, int64_t total )
 {
     for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256 )
-        out[i] = scl( in[i], filter_h( f, filter_k2( f, i ) ) );
+    {
+        float       kd = 0.f;
+        const float k2 = filter_k2( f, i, kd );
+        out[i]         = apply_h( in[i], filter_h( f, k2, kd ), f.kind );
+    }
 }
 
 // partial[b * S + s] = sum over this block's bins in shell s of Re(conj(G_i) X_i), in FP64

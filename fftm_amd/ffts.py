@@ -504,8 +504,10 @@ class ffts:
             device_ptr(shell_coeffs) if shell_coeffs is not None else None, ns, 1 if self._options.sync else 0),
             "fftm_mi_ffts_spectral_multiply")
 
-    def _op(self, kind, scale, sigma, coeffs):
-        kinds = {"poisson": _lib.OP_POISSON, "gaussian": _lib.OP_GAUSSIAN, "shell": _lib.OP_SHELL}
+    def _op(self, kind, scale, sigma, coeffs, axis=None, band=None):
+        kinds = {"poisson": _lib.OP_POISSON, "gaussian": _lib.OP_GAUSSIAN, "shell": _lib.OP_SHELL,
+                 "derivative": _lib.OP_DERIVATIVE, "poisson_gradient": _lib.OP_POISSON_GRADIENT,
+                 "bandpass": _lib.OP_BANDPASS}
         if kind not in kinds:
             raise ValueError(f"spectral operator kind must be one of {sorted(kinds)}")
         op = _lib.SpectralOp()
@@ -514,6 +516,9 @@ class ffts:
         op.sigma = float(sigma or 0.0)
         op.nshells = 0
         op.coeffs = None
+        op.axis = int(axis) if axis is not None else -1
+        if band is not None:
+            op.klo, op.khi = float(band[0]), float(band[1])
         if kind == "shell":
             if coeffs is None or coeffs.dtype != self.real or not coeffs.is_cuda or coeffs.dim() != 1:
                 raise ValueError("shell operator: coeffs must be a 1D GPU tensor of the real type")
@@ -521,24 +526,28 @@ class ffts:
             op.coeffs = device_ptr(coeffs)
         return op
 
-    def spectral_apply(self, inp, out, kind, scale=1.0, sigma=None, coeffs=None):
+    def spectral_apply(self, inp, out, kind, scale=1.0, sigma=None, coeffs=None, axis=None, band=None):
         """out = H(k) * inp with an analytic operator: "poisson" (-scale/|k|^2, 0 at k = 0),
-        "gaussian" (scale * exp(-|k|^2 / (2 sigma^2))) or "shell" (scale * coeffs[shell(k)])."""
+        "gaussian" (scale * exp(-|k|^2 / (2 sigma^2))), "shell" (scale * coeffs[shell(k)]),
+        "derivative" (i scale k_axis), "poisson_gradient" (-i scale k_axis / |k|^2) or
+        "bandpass" (scale for band[0] <= |k| <= band[1]).  axis is the logical axis
+        (x, y, z[, w] = 0, 1, 2[, 3])."""
         self._need_handle("spectral_apply")
         self._check(inp, self.complex_shape(), self.complex, "spectral_apply input")
         self._check(out, self.complex_shape(), self.complex, "spectral_apply output")
-        op = self._op(kind, scale, sigma, coeffs)
+        op = self._op(kind, scale, sigma, coeffs, axis, band)
         _lib.check(_lib.lib().fftm_mi_ffts_spectral_apply(self._handle, device_ptr(inp), device_ptr(out),
                                                           ctypes.byref(op), 1 if self._options.sync else 0),
                    "fftm_mi_ffts_spectral_apply")
 
-    def filter(self, inp, out, kind, scale=1.0, sigma=None, coeffs=None):
+    def filter(self, inp, out, kind, scale=1.0, sigma=None, coeffs=None, axis=None, band=None):
         """out = backward(H * forward(inp)) in one call (the last forward axis, the multiply and
-        the first backward axis fused into one pass); scale folds in the 1/N normalisation."""
+        the first backward axis fused into one pass); scale folds in the 1/N normalisation.
+        Operators as for spectral_apply."""
         self._need_handle("filter")
         self._check(inp, self.real_shape(), self.real, "filter input")
         self._check(out, self.real_shape(), self.real, "filter output")
-        op = self._op(kind, scale, sigma, coeffs)
+        op = self._op(kind, scale, sigma, coeffs, axis, band)
         with self._profiler.scope("ffts::filter"):
             _lib.check(_lib.lib().fftm_mi_ffts_filter(self._handle, device_ptr(inp), device_ptr(out),
                                                       ctypes.byref(op), 1 if self._options.sync else 0),

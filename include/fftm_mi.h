@@ -213,16 +213,24 @@ int fftm_mi_ffts_spectral_multiply( fftm_mi_ffts h, const void *in, void *out, c
                                     const void *shell_coeffs, int32_t nshells, int32_t sync );
 
 /*
- * Analytic diagonal operators on the half spectrum (k as for the shell filter):
- *   FFTM_MI_OP_POISSON   H = -scale / |k|^2, H(0) = 0   (periodic inverse Laplacian,
- *                        the solve_poisson_3d_functor of
- *                        source/tests/detail/versioned_ffts_3d_test_driver.h:128-154)
- *   FFTM_MI_OP_GAUSSIAN  H = scale * exp(-|k|^2 / (2 sigma^2))
- *   FFTM_MI_OP_SHELL     H = scale * coeffs[shell(k)]   (coeffs: device T[nshells])
+ * Analytic diagonal operators on the half spectrum (k as for the shell filter; k_a the
+ * wavenumber along logical axis a, Nyquist positive as in the reference drivers):
+ *   FFTM_MI_OP_POISSON          H = -scale / |k|^2, H(0) = 0   (periodic inverse
+ *                               Laplacian, solve_poisson_3d_functor of
+ *                               source/tests/detail/versioned_ffts_3d_test_driver.h:128-154)
+ *   FFTM_MI_OP_GAUSSIAN         H = scale * exp(-|k|^2 / (2 sigma^2))
+ *   FFTM_MI_OP_SHELL            H = scale * coeffs[shell(k)]   (coeffs: device T[nshells])
+ *   FFTM_MI_OP_DERIVATIVE       H = i scale k_a   (poisson_3d_derivative_spectra_functor,
+ *                               versioned_ffts_3d_test_driver.h:156-184)
+ *   FFTM_MI_OP_POISSON_GRADIENT H = -i scale k_a / |k|^2, H(0) = 0 (the two above in one)
+ *   FFTM_MI_OP_BANDPASS         H = scale where klo <= |k| <= khi, else 0
  */
 #define FFTM_MI_OP_POISSON 1
 #define FFTM_MI_OP_GAUSSIAN 2
 #define FFTM_MI_OP_SHELL 3
+#define FFTM_MI_OP_DERIVATIVE 4
+#define FFTM_MI_OP_POISSON_GRADIENT 5
+#define FFTM_MI_OP_BANDPASS 6
 typedef struct fftm_mi_spectral_op
 {
     int32_t     kind;
@@ -230,6 +238,10 @@ typedef struct fftm_mi_spectral_op
     double      scale;   /* overall factor, e.g. 1/N to normalise a round trip   */
     double      sigma;   /* OP_GAUSSIAN: width in wavenumber units (> 0)         */
     const void *coeffs;  /* OP_SHELL: device array of nshells values of type T  */
+    int32_t     axis;    /* OP_DERIVATIVE / OP_POISSON_GRADIENT: logical axis    */
+    int32_t     reserved;
+    double      klo;     /* OP_BANDPASS: band edges in wavenumber units          */
+    double      khi;
 } fftm_mi_spectral_op;
 
 /* spectrum_out = H spectrum_in on the handle's spectrum layout (in == out allowed). */

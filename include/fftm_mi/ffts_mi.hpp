@@ -103,15 +103,21 @@ struct spectral_op
     double   sigma   = 0.0;                // Gaussian width (wavenumber units)
     const T *coeffs  = nullptr;            // shell coefficients (device)
     int      nshells = 0;
+    int      axis    = -1;                 // derivative axis (logical)
+    double   klo = 0.0, khi = 0.0;         // band-pass edges
 
     fftm_mi_spectral_op c() const
     {
         fftm_mi_spectral_op o;
-        o.kind    = kind;
-        o.nshells = nshells;
-        o.scale   = scale;
-        o.sigma   = sigma;
-        o.coeffs  = coeffs;
+        o.kind     = kind;
+        o.nshells  = nshells;
+        o.scale    = scale;
+        o.sigma    = sigma;
+        o.coeffs   = coeffs;
+        o.axis     = axis;
+        o.reserved = 0;
+        o.klo      = klo;
+        o.khi      = khi;
         return o;
     }
 };

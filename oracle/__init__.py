@@ -451,14 +451,20 @@ def realaxis_weights_logical(dims) -> np.ndarray:
     return np.broadcast_to(w, tuple(dims[:-1]) + (n // 2 + 1,))
 
 
-def spectral_operator_logical(dims, kind: str, scale: float = 1.0, sigma: float = 0.0, coeffs=None) -> np.ndarray:
+def spectral_operator_logical(dims, kind: str, scale: float = 1.0, sigma: float = 0.0, coeffs=None,
+                              axis: int = 0, band=(0.0, 0.0)) -> np.ndarray:
     """H on the logical half spectrum (last axis halved), FP64:
     poisson  -scale/|k|^2 with H(0) = 0 (solve_poisson_3d_functor,
              versioned_ffts_3d_test_driver.h:128-154),
     gaussian scale*exp(-|k|^2/(2 sigma^2)),
-    shell    scale*coeffs[shell(k)].
-    Pinned in tests by closed forms: the periodic Poisson problem of
-    versioned_ffts_3d_test_driver.h:187-209 and Gaussian-filtered plane waves."""
+    shell    scale*coeffs[shell(k)],
+    derivative        i*scale*k_axis (poisson_3d_derivative_spectra_functor,
+                      versioned_ffts_3d_test_driver.h:156-184),
+    poisson_gradient  -i*scale*k_axis/|k|^2, 0 at k = 0,
+    bandpass          scale where band[0] <= |k| <= band[1].
+    Pinned in tests by closed forms: the periodic Poisson problem and its gradient
+    (versioned_ffts_3d_test_driver.h:187-209), derivatives of sine products and
+    Gaussian- or band-filtered plane waves."""
     d = len(dims)
     ks = [wavenumbers(n, a == d - 1) for a, n in enumerate(dims)]
     grids = np.meshgrid(*ks, indexing="ij")
@@ -474,6 +480,13 @@ def spectral_operator_logical(dims, kind: str, scale: float = 1.0, sigma: float 
     if kind == "shell":
         c = np.asarray(coeffs, dtype=np.float64)
         return scale * c[shell_index_logical(dims, len(c))]
+    if kind == "derivative":
+        return 1j * scale * grids[axis].astype(np.float64)
+    if kind == "poisson_gradient":
+        ka = grids[axis].astype(np.float64)
+        return np.where(k2 > 0, -1j * scale * ka / np.where(k2 > 0, k2, 1.0), 0.0)
+    if kind == "bandpass":
+        return np.where((k2 >= band[0] ** 2) & (k2 <= band[1] ** 2), scale, 0.0)
     raise ValueError(kind)
 
 

@@ -70,11 +70,13 @@ def test_gaussian_plane_waves(gpu, dt):
     f.close()
 
 
-@pytest.mark.parametrize("kind", ["poisson", "gaussian", "shell"])
+@pytest.mark.parametrize("kind,axis", [("poisson", None), ("gaussian", None), ("shell", None), ("derivative", 1),
+                                       ("derivative", 0), ("derivative", 2), ("poisson_gradient", 0),
+                                       ("bandpass", None)])
 @pytest.mark.parametrize("dims", [(128, 128, 128), (256, 64, 32), (30, 20, 24)])
-def test_filter_vs_oracle_and_unfused(gpu, kind, dims):
+def test_filter_vs_oracle_and_unfused(gpu, kind, axis, dims):
     import torch
-    print(f"case filter {kind} {dims}", flush=True)
+    print(f"case filter {kind} axis={axis} {dims}", flush=True)
     x = oracle.fill_random_real_3d(*dims, 0x5EED, np.float32)
     f = _ffts(dims)
     coeffs = None
@@ -83,24 +85,72 @@ def test_filter_vs_oracle_and_unfused(gpu, kind, dims):
         cf = np.random.default_rng(3).uniform(0.2, 2.0, size=32)
         coeffs = torch.from_numpy(cf.astype(np.float32)).cuda()
     sigma = 8.0 if kind == "gaussian" else None
+    band = (4.0, 11.5) if kind == "bandpass" else None
     scale = 1.0 / np.prod(dims)
+    kw = dict(scale=scale, sigma=sigma, coeffs=coeffs, axis=axis, band=band)
     u = torch.from_numpy(x).cuda()
     out = torch.empty_like(u)
-    f.filter(u, out, kind, scale=scale, sigma=sigma, coeffs=coeffs)
-    H = oracle.spectral_operator_logical(dims, kind, scale, sigma or 0.0, cf)
+    f.filter(u, out, kind, **kw)
+    H = oracle.spectral_operator_logical(dims, kind, scale, sigma or 0.0, cf, axis or 0, band or (0.0, 0.0))
     ref = oracle.filter_logical(x.astype(np.float64), H)
     check_close(from_gpu(out), ref, np.float32, f"filter {kind} {dims}")
     # forward + spectral_apply + backward: the same arithmetic, bitwise
     X = f.complex_array()
     f.forward(u, X)
-    f.spectral_apply(X, X, kind, scale=scale, sigma=sigma, coeffs=coeffs)
+    f.spectral_apply(X, X, kind, **kw)
     y = f.real_array()
     f.backward(X, y)
     assert torch.equal(y, out)
     # in place (real_in == real_out)
     v = u.clone()
-    f.filter(v, v, kind, scale=scale, sigma=sigma, coeffs=coeffs)
+    f.filter(v, v, kind, **kw)
     assert torch.equal(v, out)
+    f.close()
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_derivatives_and_poisson_gradient_closed_form(gpu, dt):
+    """The reference's v4 check (versioned_ffts_3d_test_driver.h:640-700): u = sin x sin y sin z,
+    rhs = -3 u; du/dx_a through the derivative operator and through the fused Poisson
+    gradient of rhs both equal cos on axis a times sin on the others."""
+    import torch
+    dims = (64, 32, 48)
+    L = [np.arange(n) * 2 * math.pi / n for n in dims]
+    X, Y, Z = np.meshgrid(*L, indexing="ij")
+    s = [np.sin(X), np.sin(Y), np.sin(Z)]
+    c = [np.cos(X), np.cos(Y), np.cos(Z)]
+    u = s[0] * s[1] * s[2]
+    npdt = np.float32 if dt == "float32" else np.float64
+    f = _ffts(dims, dt)
+    ug = torch.from_numpy(u.astype(npdt)).cuda()
+    rg = torch.from_numpy((-3.0 * u).astype(npdt)).cuda()
+    out = torch.empty_like(ug)
+    tol = 2e-6 if dt == "float32" else 1e-12
+    # i k_a multiplies the transform's roundoff at wavenumber k by k: bound by k_max = n_a / 2
+    for a in range(3):
+        exact = np.prod([c[b] if b == a else s[b] for b in range(3)], axis=0)
+        f.filter(ug, out, "derivative", scale=1.0 / np.prod(dims), axis=a)
+        e1 = np.abs(from_gpu(out) - exact).max()
+        f.filter(rg, out, "poisson_gradient", scale=1.0 / np.prod(dims), axis=a)
+        e2 = np.abs(from_gpu(out) - exact).max()
+        print(f"[derivative] axis {a} {dt}: {e1:.2e}, poisson gradient {e2:.2e}", flush=True)
+        assert e1 < tol * dims[a] / 2 and e2 < tol
+    f.close()
+
+
+def test_bandpass_plane_waves(gpu):
+    import torch
+    dims = (64, 64, 64)
+    L = [np.arange(n) * 2 * math.pi / n for n in dims]
+    X, Y, Z = np.meshgrid(*L, indexing="ij")
+    waves = [(1, 0, 0), (3, 4, 0), (0, 6, 8), (12, 0, 5), (20, 20, 20)]   # |k| = 1, 5, 10, 13, 34.6
+    src = sum(np.cos(a * X + b * Y + c * Z) for a, b, c in waves)
+    keep = sum(np.cos(a * X + b * Y + c * Z) for a, b, c in waves if 5 <= math.sqrt(a * a + b * b + c * c) <= 13)
+    f = _ffts(dims)
+    u = torch.from_numpy(src.astype(np.float32)).cuda()
+    out = torch.empty_like(u)
+    f.filter(u, out, "bandpass", scale=1.0 / np.prod(dims), band=(5.0, 13.0))
+    assert np.abs(from_gpu(out) - keep).max() < 5e-6
     f.close()
 
 

@@ -299,3 +299,34 @@ def test_rel_errors_metric():
     a = e + np.array([0.0, 1e-3, 0.0])
     rms, mx = oracle.rel_errors(a, e)
     assert abs(rms - 1e-3 / math.sqrt(14)) < 1e-15 and abs(mx - 1e-3 / 3) < 1e-15
+
+
+def test_oracle_spectral_operators_closed_forms():
+    """spectral_operator_logical + filter_logical against closed forms: the periodic
+    Poisson problem and its gradient (versioned_ffts_3d_test_driver.h:187-209), first
+    derivatives of a sine product, Gaussian- and band-filtered plane waves."""
+    import math
+    dims = (16, 12, 20)
+    N = float(np.prod(dims))
+    L = [np.arange(n) * 2 * math.pi / n for n in dims]
+    X, Y, Z = np.meshgrid(*L, indexing="ij")
+    s = [np.sin(X), np.sin(Y), np.sin(Z)]
+    c = [np.cos(X), np.cos(Y), np.cos(Z)]
+    u = s[0] * s[1] * s[2]
+    rhs = -3.0 * u
+    sol = oracle.filter_logical(rhs, oracle.spectral_operator_logical(dims, "poisson", 1.0 / N))
+    assert np.abs(sol - u).max() < 1e-12
+    for a in range(3):
+        exact = np.prod([c[b] if b == a else s[b] for b in range(3)], axis=0)
+        d = oracle.filter_logical(u, oracle.spectral_operator_logical(dims, "derivative", 1.0 / N, axis=a))
+        g = oracle.filter_logical(rhs, oracle.spectral_operator_logical(dims, "poisson_gradient", 1.0 / N, axis=a))
+        assert np.abs(d - exact).max() < 1e-12
+        assert np.abs(g - exact).max() < 1e-12
+    waves = [(1, 2, 3), (0, 5, 4), (7, 0, 9)]
+    src = sum(np.cos(a * X + b * Y + cc * Z) for a, b, cc in waves)
+    g = oracle.filter_logical(src, oracle.spectral_operator_logical(dims, "gaussian", 1.0 / N, sigma=3.0))
+    ge = sum(math.exp(-(a * a + b * b + cc * cc) / 18.0) * np.cos(a * X + b * Y + cc * Z) for a, b, cc in waves)
+    assert np.abs(g - ge).max() < 1e-12
+    bp = oracle.filter_logical(src, oracle.spectral_operator_logical(dims, "bandpass", 1.0 / N, band=(4.0, 7.0)))
+    be = np.cos(5 * Y + 4 * Z)  # |k| = 6.4 only
+    assert np.abs(bp - be).max() < 1e-12
