@@ -10,16 +10,19 @@ namespace
 template <typename T, int N>
 hipError_t launch_plane_n( const plane_params<T> &p, bool fwd, hipStream_t s )
 {
-    using G = plane_geom<N, N, (int)sizeof( cpx<T> )>;
+    // planes of 64 x 64 and smaller: two planes per workgroup share the barriers
+    constexpr int PPW = N <= 64 ? 2 : 1;
+    using G           = plane_geom<N, N, (int)sizeof( cpx<T> ), PPW>;
     if ( p.nplanes <= 0 )
         return hipSuccess;
-    if ( p.nplanes > 0x7fffffffLL )
+    const int64_t groups = ( p.nplanes + PPW - 1 ) / PPW;
+    if ( groups > 0x7fffffffLL )
         return hipErrorInvalidConfiguration;
-    const dim3 grid( (unsigned)p.nplanes ), block( G::NT );
+    const dim3 grid( (unsigned)groups ), block( G::NT );
     if ( fwd )
-        hipLaunchKernelGGL( ( k_plane<T, N, N, true> ), grid, block, 0, s, p );
+        hipLaunchKernelGGL( ( k_plane<T, N, N, true, PPW> ), grid, block, 0, s, p );
     else
-        hipLaunchKernelGGL( ( k_plane<T, N, N, false> ), grid, block, 0, s, p );
+        hipLaunchKernelGGL( ( k_plane<T, N, N, false, PPW> ), grid, block, 0, s, p );
     return hipGetLastError();
 }
 

@@ -36,7 +36,9 @@ struct plane_params
     T             scale;
 };
 
-template <int NA, int NB, int CS>
+// PPW planes per workgroup share every barrier (small planes do little work between
+// barriers); each plane owns an LDS region of ELEMS values.
+template <int NA, int NB, int CS, int PPW = 1>
 struct plane_geom
 {
     static constexpr int NAH   = NA / 2 + 1;
@@ -46,10 +48,10 @@ struct plane_geom
     static constexpr int ZELEM = ( NB / 2 ) * PZ;
     static constexpr int SELEM = NB * PA;
     static constexpr int ELEMS = ZELEM > SELEM ? ZELEM : SELEM;
-    static constexpr int TWOFF = ELEMS;                 // twiddle tables (NA, then NB entries)
-    static constexpr int TOTAL = ELEMS + NA + NB;
+    static constexpr int TWOFF = PPW * ELEMS;           // twiddle tables (NA, then NB entries)
+    static constexpr int TOTAL = PPW * ELEMS + NA + NB;
     static constexpr int BYTES = TOTAL * CS;
-    static constexpr int NT    = ELEMS * CS > 32768 ? 512 : 256;
+    static constexpr int NT    = PPW * ELEMS * CS > 32768 ? 512 : 256;
 };
 
 constexpr int plane_lds_limit() { return 163840; }
@@ -57,11 +59,14 @@ constexpr int plane_lds_limit() { return 163840; }
 // In-LDS Stockham transform of NL lines of length N.  ROWS: the line elements are
 // contiguous (element stride 1, line stride LS) and lanes walk along a line; otherwise
 // lanes walk across lines (line stride 1, element stride ES).
-template <typename T, int N, int NL, bool ROWS, bool INV, int NT, int PSH, int S = 0>
-__device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, const cpx<T> *tw )
+template <typename T, int N, int NL, bool ROWS, bool INV, int NT, int PSH, int LPG, int S = 0>
+__device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, int gs, const cpx<T> *tw )
 {
-    // element e of a line sits at e + (e >> PSH) (PSH = 0: unpadded)
-    auto at = [&]( int line, int e ) { return line * ls + ( PSH ? e + ( e >> PSH ) : e ) * es; };
+    // element e of a line sits at e + (e >> PSH) (PSH = 0: unpadded); lines come in
+    // groups of LPG (one group per plane), groups gs apart
+    auto at = [&]( int line, int e ) {
+        return ( line / LPG ) * gs + ( line % LPG ) * ls + ( PSH ? e + ( e >> PSH ) : e ) * es;
+    };
     constexpr rplan RP = make_rplan( N );
     if constexpr ( S < RP.n )
     {
@@ -110,7 +115,7 @@ __device__ __forceinline__ void lds_lines( cpx<T> *buf, int es, int ls, const cp
             }
         }
         __syncthreads();
-        lds_lines<T, N, NL, ROWS, INV, NT, PSH, S + 1>( buf, es, ls, tw );
+        lds_lines<T, N, NL, ROWS, INV, NT, PSH, LPG, S + 1>( buf, es, ls, gs, tw );
     }
 }
 
@@ -126,83 +131,95 @@ __device__ __forceinline__ void plane_offsets( const plane_params<T> &p, int64_t
 // FWD: real -> spectrum (R2C along A, C2C forward along B); else the inverse.
 // Every HBM phase is unrolled so that a thread issues all of its loads before the
 // first LDS write; the twiddle tables are copied to LDS with the first loads.
-template <typename T, int NA, int NB, bool FWD>
-__global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::NT ) )
+template <typename T, int NA, int NB, bool FWD, int PPW>
+__global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), PPW>::NT ) )
     k_plane( const plane_params<T> p )
 {
     using C                 = cpx<T>;
-    using G                 = plane_geom<NA, NB, (int)sizeof( C )>;
+    using G                 = plane_geom<NA, NB, (int)sizeof( C ), PPW>;
     constexpr int NT        = G::NT;
     constexpr int NAH       = G::NAH;
     constexpr int PA        = G::PA;
     constexpr int PZ        = G::PZ;
     constexpr int ZSH       = G::ZSH;
+    constexpr int EL        = G::ELEMS;
     constexpr int NP        = NB / 2;
     static_assert( NB % 2 == 0, "plane pass pairs the rows of B" );
     static_assert( G::BYTES <= plane_lds_limit(), "plane does not fit in LDS" );
+    static_assert( G::ZELEM <= EL && G::SELEM <= EL, "plane region" );
     __shared__ C  lds[G::TOTAL];
     C            *twa   = lds + G::TWOFF;
     C            *twb   = twa + NA;
     const int     tid   = threadIdx.x;
-    const int64_t plane = (int64_t)xcd_tile( blockIdx.x, gridDim.x );
-    int64_t       ro, co;
-    plane_offsets( p, plane, ro, co );
-    auto zi = []( int q, int n ) { return q * PZ + n + ( n >> ZSH ); }; // Z[q][n]
+    const int64_t first = (int64_t)xcd_tile( blockIdx.x, gridDim.x ) * PPW;
+    int64_t       ro[PPW], co[PPW];
+    bool          pv[PPW];
+#pragma unroll
+    for ( int g = 0; g < PPW; ++g )
+    {
+        pv[g] = first + g < p.nplanes;
+        plane_offsets( p, pv[g] ? first + g : 0, ro[g], co[g] );
+    }
+    auto zi = []( int g, int q, int n ) { return g * EL + q * PZ + n + ( n >> ZSH ); }; // Z[q][n] of plane g
 
     // twiddle tables into LDS (read once per workgroup, from L2)
     for ( int e = tid; e < NA + NB; e += NT )
         twa[e] = e < NA ? p.twa[e] : p.twb[e - NA];
 
-    constexpr int ITZ = NP * NA;  // Z elements (pairs of reals)
-    constexpr int PZT = ( ITZ + NT - 1 ) / NT;
-    constexpr int ITS = NB * NAH; // S elements (half-spectrum values)
-    constexpr int PST = ( ITS + NT - 1 ) / NT;
+    constexpr int ITZ = NP * NA;  // Z elements (pairs of reals) per plane
+    constexpr int PZT = ( PPW * ITZ + NT - 1 ) / NT;
+    constexpr int ITS = NB * NAH; // S elements (half-spectrum values) per plane
+    constexpr int PST = ( PPW * ITS + NT - 1 ) / NT;
 
     if constexpr ( FWD )
     {
-        const T *src = static_cast<const T *>( p.in ) + ro;
         // 1. Z[q][n] = (x[2q][n], x[2q+1][n])
         T ua[PZT], ub[PZT];
 #pragma unroll
         for ( int i = 0; i < PZT; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITZ % NT == 0 || e < ITZ )
+            const int e = tid + i * NT, g = e / ITZ, r = e % ITZ;
+            if ( ( ( PPW * ITZ ) % NT == 0 || e < PPW * ITZ ) && pv[g] )
             {
-                const int     q   = p.ra == 1 ? e / NA : e % NP;
-                const int     n   = p.ra == 1 ? e % NA : e / NP;
+                const T      *src = static_cast<const T *>( p.in ) + ro[g];
+                const int     q   = p.ra == 1 ? r / NA : r % NP;
+                const int     n   = p.ra == 1 ? r % NA : r / NP;
                 const int64_t off = (int64_t)n * p.ra + (int64_t)( 2 * q ) * p.rb;
                 ua[i]             = src[off];
                 ub[i]             = src[off + p.rb];
+            }
+            else
+            {
+                ua[i] = ub[i] = (T)0;
             }
         }
 #pragma unroll
         for ( int i = 0; i < PZT; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITZ % NT == 0 || e < ITZ )
+            const int e = tid + i * NT, g = e / ITZ, r = e % ITZ;
+            if ( ( PPW * ITZ ) % NT == 0 || e < PPW * ITZ )
             {
-                const int q = p.ra == 1 ? e / NA : e % NP;
-                const int n = p.ra == 1 ? e % NA : e / NP;
-                lds[zi( q, n )] = mk<T>( ua[i], ub[i] );
+                const int q = p.ra == 1 ? r / NA : r % NP;
+                const int n = p.ra == 1 ? r % NA : r / NP;
+                lds[zi( g, q, n )] = mk<T>( ua[i], ub[i] );
             }
         }
         __syncthreads();
         // 2. rows of Z
-        lds_lines<T, NA, NP, true, false, NT, ZSH>( lds, 1, PZ, twa );
+        lds_lines<T, NA, PPW * NP, true, false, NT, ZSH, NP>( lds, 1, PZ, EL, twa );
         // 3. separate pairs: S[2q][k] = A, S[2q+1][k] = B   (S aliases Z)
         constexpr int ITEMS = NP * NAH;
-        constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
+        constexpr int PER   = ( PPW * ITEMS + NT - 1 ) / NT;
         C             a[PER], b[PER];
 #pragma unroll
         for ( int i = 0; i < PER; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITEMS % NT == 0 || e < ITEMS )
+            const int e = tid + i * NT, g = e / ITEMS, r = e % ITEMS;
+            if ( ( PPW * ITEMS ) % NT == 0 || e < PPW * ITEMS )
             {
-                const int q = e / NAH, k = e % NAH;
-                const C   zk = lds[zi( q, k )];
-                const C   zm = lds[zi( q, k == 0 ? 0 : NA - k )];
+                const int q = r / NAH, k = r % NAH;
+                const C   zk = lds[zi( g, q, k )];
+                const C   zm = lds[zi( g, q, k == 0 ? 0 : NA - k )];
                 const T   h  = ( p.wmode == 1 && k != 0 && 2 * k != NA ) ? (T)1 : (T)0.5;
                 a[i]         = mk<T>( h * ( zk.x + zm.x ), h * ( zk.y - zm.y ) );
                 b[i]         = mk<T>( h * ( zk.y + zm.y ), h * ( zm.x - zk.x ) );
@@ -212,73 +229,74 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
 #pragma unroll
         for ( int i = 0; i < PER; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITEMS % NT == 0 || e < ITEMS )
+            const int e = tid + i * NT, g = e / ITEMS, r = e % ITEMS;
+            if ( ( PPW * ITEMS ) % NT == 0 || e < PPW * ITEMS )
             {
-                const int q = e / NAH, k = e % NAH;
-                lds[( 2 * q ) * PA + k]     = a[i];
-                lds[( 2 * q + 1 ) * PA + k] = b[i];
+                const int q = r / NAH, k = r % NAH;
+                lds[g * EL + ( 2 * q ) * PA + k]     = a[i];
+                lds[g * EL + ( 2 * q + 1 ) * PA + k] = b[i];
             }
         }
         __syncthreads();
         // 4. columns of S
-        lds_lines<T, NB, NAH, false, false, NT, 0>( lds, PA, 1, twb );
+        lds_lines<T, NB, PPW * NAH, false, false, NT, 0, NAH>( lds, PA, 1, EL, twb );
         // 5. store
-        C *dst = static_cast<C *>( p.out ) + co;
 #pragma unroll
         for ( int i = 0; i < PST; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITS % NT == 0 || e < ITS )
+            const int e = tid + i * NT, g = e / ITS, r = e % ITS;
+            if ( ( ( PPW * ITS ) % NT == 0 || e < PPW * ITS ) && pv[g] )
             {
-                const int bb = p.ca == 1 ? e / NAH : e % NB;
-                const int k  = p.ca == 1 ? e % NAH : e / NB;
-                dst[(int64_t)k * p.ca + (int64_t)bb * p.cb] = scl( lds[bb * PA + k], p.scale );
+                const int bb = p.ca == 1 ? r / NAH : r % NB;
+                const int k  = p.ca == 1 ? r % NAH : r / NB;
+                C        *dst = static_cast<C *>( p.out ) + co[g];
+                dst[(int64_t)k * p.ca + (int64_t)bb * p.cb] = scl( lds[g * EL + bb * PA + k], p.scale );
             }
         }
     }
     else
     {
-        const C *src = static_cast<const C *>( p.in ) + co;
         // 1. load S
         C u[PST];
 #pragma unroll
         for ( int i = 0; i < PST; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITS % NT == 0 || e < ITS )
+            const int e = tid + i * NT, g = e / ITS, r = e % ITS;
+            u[i]        = mk<T>( 0, 0 );
+            if ( ( ( PPW * ITS ) % NT == 0 || e < PPW * ITS ) && pv[g] )
             {
-                const int bb = p.ca == 1 ? e / NAH : e % NB;
-                const int k  = p.ca == 1 ? e % NAH : e / NB;
-                u[i]         = src[(int64_t)k * p.ca + (int64_t)bb * p.cb];
+                const int bb  = p.ca == 1 ? r / NAH : r % NB;
+                const int k   = p.ca == 1 ? r % NAH : r / NB;
+                const C  *src = static_cast<const C *>( p.in ) + co[g];
+                u[i]          = src[(int64_t)k * p.ca + (int64_t)bb * p.cb];
             }
         }
 #pragma unroll
         for ( int i = 0; i < PST; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITS % NT == 0 || e < ITS )
+            const int e = tid + i * NT, g = e / ITS, r = e % ITS;
+            if ( ( PPW * ITS ) % NT == 0 || e < PPW * ITS )
             {
-                const int bb = p.ca == 1 ? e / NAH : e % NB;
-                const int k  = p.ca == 1 ? e % NAH : e / NB;
-                lds[bb * PA + k] = u[i];
+                const int bb = p.ca == 1 ? r / NAH : r % NB;
+                const int k  = p.ca == 1 ? r % NAH : r / NB;
+                lds[g * EL + bb * PA + k] = u[i];
             }
         }
         __syncthreads();
         // 2. columns of S, inverse
-        lds_lines<T, NB, NAH, false, true, NT, 0>( lds, PA, 1, twb );
+        lds_lines<T, NB, PPW * NAH, false, true, NT, 0, NAH>( lds, PA, 1, EL, twb );
         // 3. Z[q][n] = A~[n] + i B~[n] from rows 2q, 2q+1 (Hermitian extension)
         C z[PZT];
 #pragma unroll
         for ( int i = 0; i < PZT; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITZ % NT == 0 || e < ITZ )
+            const int e = tid + i * NT, g = e / ITZ, r = e % ITZ;
+            if ( ( PPW * ITZ ) % NT == 0 || e < PPW * ITZ )
             {
-                const int q = e / NA, n = e % NA;
+                const int q = r / NA, n = r % NA;
                 const int nn = n <= NA / 2 ? n : NA - n;
-                C         av = lds[( 2 * q ) * PA + nn];
-                C         bv = lds[( 2 * q + 1 ) * PA + nn];
+                C         av = lds[g * EL + ( 2 * q ) * PA + nn];
+                C         bv = lds[g * EL + ( 2 * q + 1 ) * PA + nn];
                 if ( n > NA / 2 )
                 {
                     av.y = -av.y;
@@ -301,24 +319,24 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> )>::
 #pragma unroll
         for ( int i = 0; i < PZT; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITZ % NT == 0 || e < ITZ )
-                lds[zi( e / NA, e % NA )] = z[i];
+            const int e = tid + i * NT, g = e / ITZ, r = e % ITZ;
+            if ( ( PPW * ITZ ) % NT == 0 || e < PPW * ITZ )
+                lds[zi( g, r / NA, r % NA )] = z[i];
         }
         __syncthreads();
         // 4. rows of Z, inverse
-        lds_lines<T, NA, NP, true, true, NT, ZSH>( lds, 1, PZ, twa );
+        lds_lines<T, NA, PPW * NP, true, true, NT, ZSH, NP>( lds, 1, PZ, EL, twa );
         // 5. store the real rows
-        T *dst = static_cast<T *>( p.out ) + ro;
 #pragma unroll
         for ( int i = 0; i < PZT; ++i )
         {
-            const int e = tid + i * NT;
-            if ( ITZ % NT == 0 || e < ITZ )
+            const int e = tid + i * NT, g = e / ITZ, r = e % ITZ;
+            if ( ( ( PPW * ITZ ) % NT == 0 || e < PPW * ITZ ) && pv[g] )
             {
-                const int     q   = p.ra == 1 ? e / NA : e % NP;
-                const int     n   = p.ra == 1 ? e % NA : e / NP;
-                const C       v   = lds[zi( q, n )];
+                const int     q   = p.ra == 1 ? r / NA : r % NP;
+                const int     n   = p.ra == 1 ? r % NA : r / NP;
+                const C       v   = lds[zi( g, q, n )];
+                T            *dst = static_cast<T *>( p.out ) + ro[g];
                 const int64_t off = (int64_t)n * p.ra + (int64_t)( 2 * q ) * p.rb;
                 dst[off]          = v.x * p.scale;
                 dst[off + p.rb]   = v.y * p.scale;
