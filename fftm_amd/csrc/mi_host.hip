@@ -1607,7 +1607,7 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
             r.src = BUF_WORK;
     const pass_rt &lf = pre.back(), &fb = post.front();
     const bool     fusable = !lf.plane && !lf.generic && lf.kind == KIND_C2C && !lf.inv && !fb.plane && !fb.generic &&
-                         fb.kind == KIND_C2C && fb.inv && same_lines( lf, fb ) && fixed_supported( lf.N ) &&
+                         fb.kind == KIND_C2C && fb.inv && same_lines( lf, fb ) && pow2_length( lf.N ) &&
                          lf.src == BUF_WORK && lf.dst == BUF_WORK && fb.dst == BUF_WORK;
     if ( fusable )
     {
@@ -1979,7 +1979,7 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             std::vector<pass_rt> one( 1, cur );
             double               best_ms = 1e300;
             int                  best    = cur.var;
-            const int            nvar    = ( cur.plane || cur.generic ) ? 1 : NUM_VARIANTS;
+            const int            nvar    = ( cur.plane || cur.generic ) ? 1 : fixed_variants( cur.N );
             for ( int var = 0; var < nvar && rc == 0; ++var )
             {
                 if ( cur.iom && !mixed_variant_ok( var ) )

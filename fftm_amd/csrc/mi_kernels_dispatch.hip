@@ -6,6 +6,54 @@
     X( T, TN, 2 ) X( T, TN, 4 ) X( T, TN, 8 ) X( T, TN, 16 ) X( T, TN, 32 ) X( T, TN, 64 ) X( T, TN, 128 )             \
         X( T, TN, 256 ) X( T, TN, 512 ) X( T, TN, 1024 ) X( T, TN, 2048 ) X( T, TN, 4096 )
 
+// compiled lengths with factors 3, 5, 7 (must match smooth_lengths() in mi_launch.hpp)
+#define FFTM_MI_SMOOTH_LENGTHS( X, T, TN )                                                                             \
+    X( T, TN, 12 ) \
+    X( T, TN, 20 ) \
+    X( T, TN, 24 ) \
+    X( T, TN, 28 ) \
+    X( T, TN, 36 ) \
+    X( T, TN, 40 ) \
+    X( T, TN, 48 ) \
+    X( T, TN, 56 ) \
+    X( T, TN, 60 ) \
+    X( T, TN, 72 ) \
+    X( T, TN, 80 ) \
+    X( T, TN, 96 ) \
+    X( T, TN, 100 ) \
+    X( T, TN, 112 ) \
+    X( T, TN, 120 ) \
+    X( T, TN, 144 ) \
+    X( T, TN, 160 ) \
+    X( T, TN, 192 ) \
+    X( T, TN, 200 ) \
+    X( T, TN, 224 ) \
+    X( T, TN, 240 ) \
+    X( T, TN, 288 ) \
+    X( T, TN, 320 ) \
+    X( T, TN, 384 ) \
+    X( T, TN, 400 ) \
+    X( T, TN, 448 ) \
+    X( T, TN, 480 ) \
+    X( T, TN, 576 ) \
+    X( T, TN, 640 ) \
+    X( T, TN, 768 ) \
+    X( T, TN, 800 ) \
+    X( T, TN, 896 ) \
+    X( T, TN, 960 ) \
+    X( T, TN, 1000 ) \
+    X( T, TN, 1152 ) \
+    X( T, TN, 1280 ) \
+    X( T, TN, 1344 ) \
+    X( T, TN, 1400 ) \
+    X( T, TN, 1536 ) \
+    X( T, TN, 1600 ) \
+    X( T, TN, 1680 ) \
+    X( T, TN, 1792 ) \
+    X( T, TN, 1920 ) \
+    X( T, TN, 2000 ) \
+    X( T, TN, 2100 )
+
 #define FFTM_MI_DECLARE( T, TN, N )                                                                                    \
     hipError_t launch_##TN##_n##N( const pass_params<T> &, int, bool, bool, int, hipStream_t );                       \
     int        tile_cols_##TN##_n##N( bool, int );
@@ -21,12 +69,15 @@ namespace fftm_mi
 
 FFTM_MI_LENGTHS( FFTM_MI_DECLARE, float, f32 )
 FFTM_MI_LENGTHS( FFTM_MI_DECLARE, double, f64 )
+FFTM_MI_SMOOTH_LENGTHS( FFTM_MI_DECLARE, float, f32 )
+FFTM_MI_SMOOTH_LENGTHS( FFTM_MI_DECLARE, double, f64 )
 
 hipError_t launch_pass_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, hipStream_t s )
 {
     switch ( N )
     {
         FFTM_MI_LENGTHS( FFTM_MI_CASE_LAUNCH, float, f32 )
+        FFTM_MI_SMOOTH_LENGTHS( FFTM_MI_CASE_LAUNCH, float, f32 )
     default: return hipErrorInvalidValue;
     }
 }
@@ -36,6 +87,7 @@ hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool 
     switch ( N )
     {
         FFTM_MI_LENGTHS( FFTM_MI_CASE_LAUNCH, double, f64 )
+        FFTM_MI_SMOOTH_LENGTHS( FFTM_MI_CASE_LAUNCH, double, f64 )
     default: return hipErrorInvalidValue;
     }
 }
@@ -45,6 +97,7 @@ int fixed_tile_cols_f32( int N, bool loadb, int var )
     switch ( N )
     {
         FFTM_MI_LENGTHS( FFTM_MI_CASE_COLS, float, f32 )
+        FFTM_MI_SMOOTH_LENGTHS( FFTM_MI_CASE_COLS, float, f32 )
     default: return 0;
     }
 }
@@ -54,6 +107,7 @@ int fixed_tile_cols_f64( int N, bool loadb, int var )
     switch ( N )
     {
         FFTM_MI_LENGTHS( FFTM_MI_CASE_COLS, double, f64 )
+        FFTM_MI_SMOOTH_LENGTHS( FFTM_MI_CASE_COLS, double, f64 )
     default: return 0;
     }
 }

@@ -13,7 +13,7 @@ namespace detail
 template <int N>
 constexpr int variants_for()
 {
-    return N >= 64 ? NUM_VARIANTS : 1;
+    return ( N & ( N - 1 ) ) == 0 && N >= 64 ? NUM_VARIANTS : 1;
 }
 
 template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32>

@@ -11,10 +11,26 @@
 namespace fftm_mi
 {
 
-// compiled lengths of the fixed-N kernels: powers of two 2..4096
-inline bool fixed_supported( int64_t N )
+// compiled lengths of the fixed-N kernels: powers of two 2..4096 (all tile-shape
+// variants from 64 up) and a set of 2^a 3^b 5^c 7^d lengths (default tile shape)
+inline bool pow2_length( int64_t N )
 {
     return N >= 2 && N <= 4096 && ( N & ( N - 1 ) ) == 0;
+}
+inline bool fixed_supported( int64_t N )
+{
+    static const int64_t smooth[] = { 12, 20, 24, 28, 36, 40, 48, 56, 60, 72, 80, 96, 100, 112, 120, 144, 160, 192, 200, 224, 240, 288, 320, 384, 400, 448, 480, 576, 640, 768, 800, 896, 960, 1000, 1152, 1280, 1344, 1400, 1536, 1600, 1680, 1792, 1920, 2000, 2100 };
+    if ( pow2_length( N ) )
+        return true;
+    for ( int64_t v : smooth )
+        if ( v == N )
+            return true;
+    return false;
+}
+// tile-shape variants compiled for a length
+inline int fixed_variants( int64_t N )
+{
+    return pow2_length( N ) && N >= 64 ? NUM_VARIANTS : 1;
 }
 
 // columns per workgroup of a fixed-N kernel (complex lines)

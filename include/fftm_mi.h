@@ -179,7 +179,9 @@ int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, cons
  * 6 the tile of 0 exchanged through LDS one component at a time (half the LDS per
  * workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Results are bitwise
  * identical across variants; only the speed differs.  Set by the tuner or by hand.
- * FP16-storage passes (FFTM_MI_F32_H16) have variants 0, 6 and 7. */
+ * FP16-storage passes (FFTM_MI_F32_H16) have variants 0, 6 and 7.  Lengths compiled
+ * with the default shape only (below 64, and the 3/5/7-smooth lengths) and plane or
+ * any-radix passes run the default shape whatever the variant. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
 int fftm_mi_ffts_get_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t *variant );
 /* Algorithmic HBM bytes one pass moves (reads + writes of its input and output). */

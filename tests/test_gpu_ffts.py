@@ -34,7 +34,8 @@ def _strategy(name):
 
 @pytest.mark.parametrize("execution", EXECUTIONS)
 @pytest.mark.parametrize("dt", DT)
-@pytest.mark.parametrize("dims", [(12, 10, 8), (16, 16, 16), (64, 32, 48), (128, 128, 128), (9, 15, 6)])
+@pytest.mark.parametrize("dims", [(12, 10, 8), (16, 16, 16), (64, 32, 48), (128, 128, 128), (9, 15, 6),
+                                  (96, 160, 384), (480, 36, 100)])
 def test_ffts_3d_parity(gpu, dt, dims, execution):
     nx, ny, nz = dims
     print(f"case ffts3d {execution} dt={np.dtype(dt).name} dims={dims}", flush=True)
@@ -57,7 +58,7 @@ def test_ffts_3d_parity(gpu, dt, dims, execution):
 @pytest.mark.parametrize("execution", EXECUTIONS)
 @pytest.mark.parametrize("strategy", ["pencil", "slab", "xzwy"])
 @pytest.mark.parametrize("dt", DT)
-@pytest.mark.parametrize("dims", [(12, 10, 8, 10), (16, 16, 16, 16), (8, 12, 16, 6), (32, 16, 8, 24)])
+@pytest.mark.parametrize("dims", [(12, 10, 8, 10), (16, 16, 16, 16), (8, 12, 16, 6), (32, 16, 8, 24), (48, 20, 24, 40)])
 def test_ffts_4d_parity(gpu, dt, dims, strategy, execution):
     if strategy == "xzwy" and execution == "staged":
         pytest.skip("native_xzwy exists only as a fused transform")

@@ -174,22 +174,18 @@ def test_plan_c2c_exec_direction(gpu, dt):
 
 @pytest.mark.parametrize("dt", DT)
 @pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 3, 5, 6, 7, 10, 12, 14, 22,
-                               26, 30, 36, 100, 1000, 1344, 2100])
+                               26, 30, 36, 100, 1000, 1344, 2100, 48, 96, 160, 384, 480, 896, 1536, 1680, 1920,
+                               2000])
 def test_plan_1d_lengths(gpu, dt, n):
-    """Every compiled power-of-two length and mixed radices (2,3,5,7,11,13), both mappings,
-    R2C/C2R pairing with an odd batch (phantom column)."""
+    """Every compiled power-of-two length, compiled 3/5/7-smooth lengths and any-radix
+    lengths (2,3,5,7,11,13), both mappings, R2C/C2R pairing with an odd batch (phantom
+    column)."""
     import fftm_amd
     D = fftm_amd.direction
     rt, ct = torch_dtypes(dt)
     cdt = np.complex64 if dt == np.float32 else np.complex128
     nh = n // 2 + 1
     batch = 7
-    if dt == np.float64 and n == 2100:
-        # the any-radix kernel keeps two FP64 lines of 2100 in LDS: 67 KB > 64 KB
-        with pytest.raises(fftm_amd.FftmMiError) as e:
-            _plan(dt, D.R2C, [n], [n], 1, n, [nh], 1, nh, batch)
-        assert e.value.status == 16
-        return
     x = _rng_real((batch, n), n).astype(dt)
     # contiguous lines
     p = _plan(dt, D.R2C, [n], [n], 1, n, [nh], 1, nh, batch)
