@@ -234,11 +234,13 @@ class fft:
 
 
 class fft_wrap_many:
-    """Named-plan container (fft_wrap_many.h:25-1927), the subset ffts uses.
+    """Named-plan container (fft_wrap_many.h:25-1927): the plans ffts uses, raw-pointer
+    execution, plan sequences executed at element offsets (fft_wrap_many.h:435-780) and
+    plans bound to their own non-blocking streams (fft_wrap_many.h:396-433).
 
-    The fftm-only plan sequences, C2C plan arrays and R2C/C2R bundles
-    (fft_wrap_many.h:188-237, 453-1605) belong to the MPI frontend and raise
-    ``NotImplementedError``."""
+    The C2C plan arrays and R2C/C2R bundles of the MPI frontend (fft_wrap_many.h:188-237,
+    1260-1605) raise ``NotImplementedError``; C++ callers get them from the reference's
+    own container on top of ``fftm::wrap::mi::fft``."""
 
     def __init__(self, real):
         self.real = real
@@ -373,13 +375,90 @@ class fft_wrap_many:
         self._memory_profiler.set_bytes(self._memory_profile_prefix + "/work_area",
                                         0 if self._external_activated else self._work_area_size)
 
+    # -- raw execution (fft_wrap_many.h:435-451) ------------------------------
+    def exec_raw(self, name: str, in_ptr: int, out_ptr: int):
+        p = self._plans[name]
+        p._exec(p.get_direction(), int(in_ptr), int(out_ptr), sync=not self._hot_exec_no_sync)
+
+    def exec_raw_direction(self, name: str, exec_dir, in_ptr: int, out_ptr: int):
+        self._plans[name]._exec(direction(exec_dir), int(in_ptr), int(out_ptr), sync=not self._hot_exec_no_sync)
+
+    # -- owned streams (fft_wrap_many.h:396-433) -------------------------------
+    def bind_plan_to_owned_nonblocking_stream(self, name: str):
+        if not hasattr(self, "_owned_streams"):
+            self._owned_streams = {}
+        if name not in self._owned_streams:
+            self._owned_streams[name] = runtime_api.create_nonblocking_stream()
+        self._plans[name].set_stream(self._owned_streams[name])
+
+    def synchronize_owned_plan_stream(self, name: str):
+        streams = getattr(self, "_owned_streams", {})
+        if name not in streams:
+            raise KeyError("fft_wrap_many: plan has no owned execution stream")
+        runtime_api.stream_synchronize(streams[name])
+
+    # -- plan sequences (fft_wrap_many.h:453-780) -------------------------------
+    def make_plan_sequence(self, names) -> int:
+        """A fixed list of named plans, executed in order; returns its id."""
+        if not hasattr(self, "_sequences"):
+            self._sequences = []
+        self._sequences.append([self._plans[n] for n in names])
+        return len(self._sequences) - 1
+
+    def plan_sequence_size(self, sequence_id: int) -> int:
+        return len(self._sequence(sequence_id))
+
+    def _sequence(self, sequence_id):
+        seqs = getattr(self, "_sequences", [])
+        if sequence_id < 0 or sequence_id >= len(seqs):
+            raise IndexError("fft_wrap_many: invalid plan sequence id")
+        return seqs[sequence_id]
+
+    def _run_sequence(self, sequence_id, offsets, exec_dir, inp, out, sync, what):
+        seq = self._sequence(sequence_id)
+        if len(seq) != len(offsets):
+            raise LogicError(f"fft_wrap_many::{what}: plan/offset count mismatch.")
+        # the offsets count elements of the input / output arrays (typed pointer arithmetic)
+        ib, ob = inp.element_size(), out.element_size()
+        ip, op = device_ptr(inp), device_ptr(out)
+        for plan, off in zip(seq, offsets):
+            d = plan.get_direction() if exec_dir is None else direction(exec_dir)
+            plan._exec(d, ip + int(off) * ib, op + int(off) * ob, sync=sync)
+
+    def exec_plan_sequence_offsets(self, sequence_id, offsets, inp, out):
+        self._run_sequence(sequence_id, offsets, None, inp, out, True, "exec_plan_sequence_offsets")
+
+    def exec_plan_sequence_offsets_direction(self, sequence_id, offsets, exec_dir, inp, out):
+        self._run_sequence(sequence_id, offsets, exec_dir, inp, out, True, "exec_plan_sequence_offsets_direction")
+
+    def exec_plan_sequence_offsets_direction_repeated(self, sequence_id, offsets, exec_dir, inp, out, repeats):
+        for _ in range(int(repeats)):
+            self.exec_plan_sequence_offsets_direction(sequence_id, offsets, exec_dir, inp, out)
+
+    def exec_plan_sequence_offsets_no_sync(self, sequence_id, offsets, inp, out):
+        self._run_sequence(sequence_id, offsets, None, inp, out, False, "exec_plan_sequence_offsets_no_sync")
+
+    def exec_plan_sequence_offsets_direction_no_sync(self, sequence_id, offsets, exec_dir, inp, out):
+        self._run_sequence(sequence_id, offsets, exec_dir, inp, out, False,
+                           "exec_plan_sequence_offsets_direction_no_sync")
+
+    def exec_plan_sequence_offsets_direction_no_sync_repeated(self, sequence_id, offsets, exec_dir, inp, out,
+                                                              repeats):
+        for _ in range(int(repeats)):
+            self.exec_plan_sequence_offsets_direction_no_sync(sequence_id, offsets, exec_dir, inp, out)
+
+    # the reference's "tight" and "opaque" forms are faster dispatches of the same
+    # sequence (typed C2C plans, raw plan handles); here they are the same loop
+    exec_plan_sequence_offsets_tight = exec_plan_sequence_offsets
+    exec_plan_sequence_offsets_tight_direction = exec_plan_sequence_offsets_direction
+    exec_plan_sequence_offsets_opaque_direction = exec_plan_sequence_offsets_direction
+
     # -- fftm-only surface ---------------------------------------------------
     def _fftm_only(self, *a, **k):
-        raise NotImplementedError("fft_wrap_many: plan sequences, C2C plan arrays and R2C/C2R bundles belong to "
-                                  "the MPI fftm frontend and are not supported by this backend")
+        raise NotImplementedError("fft_wrap_many: C2C plan arrays and R2C/C2R bundles belong to the MPI fftm "
+                                  "frontend; use the reference's fft_wrap_many over fftm::wrap::mi::fft in C++")
 
     make_r2c_c2r_plan_bundle_2D = _fftm_only
-    make_plan_sequence = _fftm_only
     make_c2c_plan_array = _fftm_only
 
     def __del__(self):
