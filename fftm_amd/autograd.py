@@ -1,7 +1,8 @@
 """Thin torch.autograd binding of the hand-written adjoints, used to check them.
 
 ``SpectralFilter.apply(u, coeffs, f)`` computes v = C2R(H(coeffs) * R2C(u)) with a
-radial shell filter H on an initialised fused ``ffts`` instance ``f``.  Its backward
+radial shell filter H on an initialised fused ``ffts`` instance ``f`` (one
+``ffts.filter`` call).  Its backward
 uses only the C-ABI adjoint kernels and recomputes the forward R2C instead of
 storing the spectrum:
 
@@ -20,11 +21,10 @@ class SpectralFilter(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, u, coeffs, f):
-        X = f.complex_array()
-        f.forward(u.contiguous(), X)
-        f.spectral_multiply(X, X, shell_coeffs=coeffs.contiguous())
+        # one fused filter call (the spectrum stays in the work area); bitwise equal to
+        # forward + radial multiply + backward
         v = f.real_array()
-        f.backward(X, v)
+        f.filter(u.contiguous(), v, "shell", scale=1.0, coeffs=coeffs.contiguous())
         ctx.f = f
         ctx.save_for_backward(u, coeffs)
         return v
