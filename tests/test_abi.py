@@ -214,3 +214,39 @@ def test_ffts_validate_fp16_storage():
     assert L.lib().fftm_mi_ffts_validate(1, L.i64_array([128] * 4), L.F32_H16, 0, ctypes.byref(ws)) == 16
     assert L.lib().fftm_mi_ffts_validate(0, L.i64_array([128, 96, 128]), L.F32_H16, 0, ctypes.byref(ws)) == 16
     assert L.lib().fftm_mi_ffts_validate(0, L.i64_array([64, 128, 128]), L.F32_H16, 0, ctypes.byref(ws)) == 16
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/fftm_mi.h compiles as C99 (no C++ in the declarations) and a C program
+    links against the library and calls the host-only entry points."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "abi.c"
+    src.write_text(
+        '#include "fftm_mi.h"\n'
+        '#include <stdio.h>\n'
+        'int main(void) {\n'
+        '    fftm_mi_plan_desc d = {0};\n'
+        '    size_t ws = 0;\n'
+        '    int64_t n[3] = {512, 512, 512};\n'
+        '    fftm_mi_spectral_op op = {0};\n'
+        '    (void)op;\n'
+        '    d.dir = 1; d.precision = FFTM_MI_F32; d.rank = 3; d.default_layout = 1;\n'
+        '    d.n[0] = d.n[1] = d.n[2] = 64; d.batch = 1;\n'
+        '    if (fftm_mi_plan_validate(&d, &ws) != 0) return 1;\n'
+        '    if (fftm_mi_ffts_validate(FFTM_MI_LAYOUT_3D, n, FFTM_MI_F32, 0, &ws) != 0) return 2;\n'
+        '    printf("%d %zu\\n", fftm_mi_version(), ws);\n'
+        '    return 0;\n'
+        '}\n')
+    exe = tmp_path / "abi"
+    lib_dir = os.path.dirname(_lib().LIB_PATH)
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe), "-L", lib_dir, "-lfftm_mi", "-Wl,-rpath," + lib_dir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split()[1] == str(512 * 512 * 257 * 8)
