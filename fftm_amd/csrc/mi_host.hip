@@ -1994,18 +1994,37 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
                     rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, true, "autotune" );
                 if ( rc != 0 )
                     break;
-                bool valid = true;
+                // every copy, clear and comparison is ordered on the handle's stream (which
+                // may be non-blocking with respect to the null stream)
+                bool       valid = true;
+                hipError_t he    = hipSuccess;
                 if ( var == 0 )
-                    (void)hipMemcpy( ref, dstbuf, dbytes, hipMemcpyDeviceToDevice );
+                {
+                    he = hipMemcpyAsync( ref, dstbuf, dbytes, hipMemcpyDeviceToDevice, h->stream );
+                    if ( he == hipSuccess )
+                        he = hipStreamSynchronize( h->stream );
+                }
                 else
                 {
-                    (void)hipMemset( dcount, 0, sizeof( unsigned long long ) );
-                    hipLaunchKernelGGL( k_count_diff, dim3( 1024 ), dim3( 256 ), 0, h->stream,
-                                        static_cast<const uint32_t *>( ref ), static_cast<const uint32_t *>( dstbuf ),
-                                        (int64_t)( dbytes / 4 ), dcount );
                     unsigned long long diff = 0;
-                    (void)hipMemcpy( &diff, dcount, sizeof( diff ), hipMemcpyDeviceToHost );
+                    he = hipMemsetAsync( dcount, 0, sizeof( unsigned long long ), h->stream );
+                    if ( he == hipSuccess )
+                    {
+                        hipLaunchKernelGGL( k_count_diff, dim3( 1024 ), dim3( 256 ), 0, h->stream,
+                                            static_cast<const uint32_t *>( ref ), static_cast<const uint32_t *>( dstbuf ),
+                                            (int64_t)( dbytes / 4 ), dcount );
+                        he = hipGetLastError();
+                    }
+                    if ( he == hipSuccess )
+                        he = hipMemcpyAsync( &diff, dcount, sizeof( diff ), hipMemcpyDeviceToHost, h->stream );
+                    if ( he == hipSuccess )
+                        he = hipStreamSynchronize( h->stream );
                     valid = diff == 0;
+                }
+                if ( he != hipSuccess )
+                {
+                    rc = fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_ffts_autotune: " ) + hipGetErrorString( he ) );
+                    break;
                 }
                 // timing: warmup, then `iterations` single launches, median
                 std::vector<float> ms;
@@ -2013,12 +2032,18 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
                     rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, false, "autotune" );
                 for ( int it = 0; it < iterations && rc == 0; ++it )
                 {
-                    (void)hipEventRecord( e0, h->stream );
-                    rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, false, "autotune" );
-                    (void)hipEventRecord( e1, h->stream );
-                    (void)hipEventSynchronize( e1 );
                     float t = 0.f;
-                    (void)hipEventElapsedTime( &t, e0, e1 );
+                    he      = hipEventRecord( e0, h->stream );
+                    if ( he == hipSuccess )
+                        rc = run_passes( one, h->tb, h->prec, in, out, h->work, h->stream, false, false, "autotune" );
+                    if ( he == hipSuccess && rc == 0 )
+                        he = hipEventRecord( e1, h->stream );
+                    if ( he == hipSuccess && rc == 0 )
+                        he = hipEventSynchronize( e1 );
+                    if ( he == hipSuccess && rc == 0 )
+                        he = hipEventElapsedTime( &t, e0, e1 );
+                    if ( he != hipSuccess )
+                        rc = fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_ffts_autotune: " ) + hipGetErrorString( he ) );
                     ms.push_back( t );
                 }
                 if ( rc != 0 )
