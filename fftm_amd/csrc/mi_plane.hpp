@@ -51,7 +51,11 @@ struct plane_geom
     static constexpr int TWOFF = PPW * ELEMS;           // twiddle tables (NA, then NB entries)
     static constexpr int TOTAL = PPW * ELEMS + NA + NB;
     static constexpr int BYTES = TOTAL * CS;
-    static constexpr int NT    = PPW * ELEMS * CS > 32768 ? 512 : 256;
+    // threads: more waves per workgroup for the large planes (the LDS tile caps the
+    // workgroups per CU at two, so memory-level parallelism comes from the waves)
+    static constexpr int NT    = PPW * ELEMS * CS > 65536 ? 1024 : ( PPW * ELEMS * CS > 32768 ? 512 : 256 );
+    // waves per SIMD for two such workgroups per CU when the tile allows two
+    static constexpr int WPE   = ( 2 * BYTES <= 163840 && NT == 1024 ) ? 8 : 1;
 };
 
 constexpr int plane_lds_limit() { return 163840; }
@@ -133,6 +137,7 @@ __device__ __forceinline__ void plane_offsets( const plane_params<T> &p, int64_t
 // first LDS write; the twiddle tables are copied to LDS with the first loads.
 template <typename T, int NA, int NB, bool FWD, int PPW>
 __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), PPW>::NT ) )
+    __attribute__( ( amdgpu_waves_per_eu( plane_geom<NA, NB, (int)sizeof( cpx<T> ), PPW>::WPE ) ) )
     k_plane( const plane_params<T> p )
 {
     using C                 = cpx<T>;
