@@ -211,7 +211,9 @@ struct rplan
     int r[24];
 };
 
-constexpr rplan make_rplan( int N )
+// big_last moves a leading radix 16 behind the radix-8 stages (lines read from HBM in
+// pairs of consecutive elements need an even number of butterflies per thread in stage 0)
+constexpr rplan make_rplan( int N, bool big_last = false )
 {
     rplan p{ 0, {} };
     int   m = N, k = 0;
@@ -239,6 +241,12 @@ constexpr rplan make_rplan( int N )
     {
         p.r[p.n++] = 8;
         k -= 3;
+    }
+    if ( big_last && p.n > 1 && p.r[0] == 16 )
+    {
+        for ( int i = 0; i + 1 < p.n; ++i )
+            p.r[i] = p.r[i + 1];
+        p.r[p.n - 1] = 16;
     }
     for ( int f = 3; f <= m; f += 2 )
         while ( m % f == 0 )
@@ -521,7 +529,8 @@ struct pass_kernel
     static_assert( !( IOM & IO_H16_IN ) || KIND != KIND_R2C, "R2C reads real FP32 input" );
     static_assert( !( IOM & IO_H16_OUT ) || KIND != KIND_C2R, "C2R writes real FP32 output" );
     using C                    = cpx<T>;
-    static constexpr rplan RP  = make_rplan( N );
+    // R2C restages its output through LDS, so only its stage-0 loads want pairs
+    static constexpr rplan RP  = make_rplan( N, KIND == KIND_R2C );
     static constexpr bool  SPLIT = variant_split( VAR );
     // FP16-stored lines move 4 bytes per bin: their variant 7 doubles the columns (128-byte
     // FP16 row segments) with the split exchange, instead of capping registers
