@@ -1613,7 +1613,7 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
     {
         // the tile shape follows the tuned forward pass it replaces (shapes 0, 2, 4)
         pre.back().filter = true;
-        pre.back().var    = ( lf.var == 2 || lf.var == 4 ) ? lf.var : ( lf.var == 5 ? 4 : ( lf.var == 3 ? 2 : 0 ) );
+        pre.back().var    = ( lf.var == 2 || lf.var == 4 ) ? lf.var : ( ( lf.var == 3 || lf.var == 5 ) ? 4 : 0 );
         post.erase( post.begin() );
     }
     h.flt_pre     = pre;

@@ -291,7 +291,8 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 // var selects a tile-shape variant (0 = default); the tuner measures them per pass:
 //   1: half the columns per workgroup (smaller LDS tile, more workgroups per CU)
 //   2: twice the threads per column (fewer elements per thread, larger workgroup)
-//   3: both (same workgroup size, half the tile)
+//   3: the tile of 4 exchanged through LDS in two rounds (real parts, then imaginary
+//      parts): 256-byte row segments at variant 0's LDS per workgroup
 //   4: twice the columns (256-byte row segments; LDS up to 160 KB on gfx950)
 //   5: twice the columns and twice the threads per column (1024-thread workgroups)
 //   6: the tile of 0, exchanged through LDS in two rounds (real parts, then imaginary
@@ -300,8 +301,8 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 //      (some spilling; wins on the longest contiguous R2C lines)
 constexpr int NUM_VARIANTS = 8;
 
-constexpr bool variant_split( int var ) { return var >= 6; }
-constexpr int  variant_shape( int var ) { return var >= 6 ? 0 : var; }
+constexpr bool variant_split( int var ) { return var >= 6 || var == 3; }
+constexpr int  variant_shape( int var ) { return var >= 6 ? 0 : ( var == 3 ? 4 : var ); }
 
 constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 {
@@ -347,9 +348,9 @@ constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
         g.tpc = tpc;
         g.pad = pad;
     }
-    if ( ( var == 1 || var == 3 ) && g.tw > 1 )
+    if ( var == 1 && g.tw > 1 )
         g.tw /= 2;
-    if ( ( var == 2 || var == 3 ) && g.tpc * 2 <= hi && g.tpc * 2 * g.tw <= 1024 )
+    if ( var == 2 && g.tpc * 2 <= hi && g.tpc * 2 * g.tw <= 1024 )
         g.tpc *= 2;
     if ( var == 4 || var == 5 )
     {
@@ -553,7 +554,7 @@ struct pass_kernel
     // the split exchange halves the LDS tile so that more workgroups share a CU: ask the
     // register allocator for the matching occupancy (waves per SIMD)
     static constexpr int WAVES_PER_EU = [] {
-        if ( VAR != 7 || IOM != IO_F32 )
+        if ( ( VAR != 7 && VAR != 3 ) || IOM != IO_F32 )
             return 1;
         const int lds_wg  = LDS * (int)sizeof( LW );
         const int wgs     = lds_wg > 0 ? 163840 / lds_wg : 8;

@@ -174,8 +174,9 @@ int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *b
  * time individual kernels; it has no counterpart in the reference. */
 int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
 /* Tile-shape variant (0..7) of one pass's kernel: 0 default, 1 half the columns per
- * workgroup, 2 twice the threads per column, 3 both, 4 twice the columns (256-byte row
- * segments, up to 160 KB of LDS), 5 twice the columns and the threads per column,
+ * workgroup, 2 twice the threads per column, 3 the tile of 4 exchanged through LDS one
+ * component at a time, 4 twice the columns (256-byte row segments, up to 160 KB of
+ * LDS), 5 twice the columns and the threads per column,
  * 6 the tile of 0 exchanged through LDS one component at a time (half the LDS per
  * workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Results are bitwise
  * identical across variants; only the speed differs.  Set by the tuner or by hand.
