@@ -330,6 +330,10 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.wmode   = r.wmode;
     p.vec     = pair_access_bits( r, src, dst );
     p.scale   = (T)r.scale;
+    // tile shape (compile-time kernel) and tile order (runtime bit)
+    const int  shape = r.var & ~VARIANT_RR;
+    const int  rr    = ( r.var & VARIANT_RR ) ? VEC_RR : 0;
+    p.vec |= rr;
     if constexpr ( std::is_same<T, float>::value )
     {
         if ( r.iom )
@@ -344,20 +348,20 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
             p.ys0            = r.ys0;
             p.ys1            = r.ys1;
             p.ys2            = r.ys2;
-            p.vec            = 0;
-            return launch_pass_mixed_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, r.iom, s );
+            p.vec            = rr;
+            return launch_pass_mixed_f32( p, (int)r.N, r.kind, r.inv, r.loadb, shape, r.iom, s );
         }
         if ( r.filter )
-            return launch_filter_f32( p, *static_cast<const filter_params<float> *>( filter ), (int)r.N, r.loadb, r.var, s );
+            return launch_filter_f32( p, *static_cast<const filter_params<float> *>( filter ), (int)r.N, r.loadb, shape, s );
         return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
-                         : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
+                         : launch_pass_f32( p, (int)r.N, r.kind, r.inv, r.loadb, shape, s );
     }
     else
     {
         if ( r.filter )
-            return launch_filter_f64( p, *static_cast<const filter_params<double> *>( filter ), (int)r.N, r.loadb, r.var, s );
+            return launch_filter_f64( p, *static_cast<const filter_params<double> *>( filter ), (int)r.N, r.loadb, shape, s );
         return r.generic ? launch_generic_f64( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
-                         : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, r.var, s );
+                         : launch_pass_f64( p, (int)r.N, r.kind, r.inv, r.loadb, shape, s );
     }
 }
 
@@ -969,6 +973,7 @@ struct fftm_mi_ffts_s
     std::vector<pass_rt> flt_pre, flt_post;
     bool                 flt_ready = false, flt_fused = false;
     int                  flt_line_ax = -1; // logical axis of the fused filter pass's lines
+    int                  flt_var     = -1; // tuned variant of the fused filter pass (-1: from the forward pass)
 
     ~fftm_mi_ffts_s()
     {
@@ -1613,7 +1618,10 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
     {
         // the tile shape follows the tuned forward pass it replaces (shapes 0, 2, 4)
         pre.back().filter = true;
-        pre.back().var    = ( lf.var == 2 || lf.var == 4 ) ? lf.var : ( ( lf.var == 3 || lf.var == 5 ) ? 4 : 0 );
+        const int lv      = lf.var & ~VARIANT_RR;
+        pre.back().var    = h.flt_var >= 0 ? h.flt_var
+                                           : ( ( lv == 2 || lv == 4 ) ? lv : ( ( lv == 3 || lv == 5 ) ? 4 : 0 ) ) |
+                                                 ( lf.var & VARIANT_RR );
         post.erase( post.begin() );
     }
     h.flt_pre     = pre;
@@ -1697,10 +1705,10 @@ int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t ind
     std::vector<pass_rt> &v = backward ? h->bwd : h->fwd;
     if ( index < 0 || index >= (int32_t)v.size() )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: pass index out of range" );
-    if ( variant < 0 || variant >= NUM_VARIANTS )
+    if ( !variant_number_ok( variant ) )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: variant out of range" );
     if ( v[(size_t)index].iom && !mixed_variant_ok( variant ) )
-        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_set_pass_variant: FP16-storage passes have variants 0, 6, 7" );
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_set_pass_variant: FP16-storage passes have tile shapes 0, 6, 7" );
     v[(size_t)index].var = variant;
     h->flt_ready         = false;
     return ok();
@@ -1732,8 +1740,8 @@ int fftm_mi_ffts_destroy( fftm_mi_ffts h )
 // (source/fftm_autotune.hpp:126-169 KEY=VALUE cache, header line, tmp + rename;
 //  detail/fftm_autotune_hardware.inc:87-98, 285-337 FNV-1a-64 "v2:" signature;
 //  fftm_autotune_measure.hpp:293-400 median of iterations after warmup, per-candidate
-//  records, fastest valid candidate wins).  Here the candidates are the tile-shape
-//  variants of each pass; validity = output bitwise equal to variant 0.
+//  records, fastest valid candidate wins).  Here the candidates are the variants of
+//  each pass (tile shape x tile order); validity = output bitwise equal to variant 0.
 // ---------------------------------------------------------------------------
 namespace
 {
@@ -1854,6 +1862,128 @@ __global__ void k_count_diff( const uint32_t *a, const uint32_t *b, int64_t n, u
         atomicAdd( count, c );
 }
 
+// variants of the fused filter pass: tile shapes 0, 2 and 4, each in both tile orders
+constexpr int FILTER_CANDIDATES[] = { 0, 2, 4, VARIANT_RR, VARIANT_RR | 2, VARIANT_RR | 4 };
+
+bool filter_variant_ok( int var )
+{
+    for ( int c : FILTER_CANDIDATES )
+        if ( var == c )
+            return true;
+    return false;
+}
+
+// Bitwise comparison of n bytes (a multiple of 4) on the handle's stream: *equal is set
+// when they match.  `ref` receives a copy instead when `capture` is set.
+hipError_t tune_compare( const fftm_mi_ffts_s &h, void *ref, const void *buf, size_t n, bool capture,
+                         unsigned long long *dcount, bool *equal )
+{
+    *equal        = true;
+    hipError_t he = hipSuccess;
+    if ( capture )
+    {
+        he = hipMemcpyAsync( ref, buf, n, hipMemcpyDeviceToDevice, h.stream );
+        return he == hipSuccess ? hipStreamSynchronize( h.stream ) : he;
+    }
+    unsigned long long diff = 0;
+    he                      = hipMemsetAsync( dcount, 0, sizeof( unsigned long long ), h.stream );
+    if ( he == hipSuccess )
+    {
+        hipLaunchKernelGGL( k_count_diff, dim3( 1024 ), dim3( 256 ), 0, h.stream, static_cast<const uint32_t *>( ref ),
+                            static_cast<const uint32_t *>( buf ), (int64_t)( n / 4 ), dcount );
+        he = hipGetLastError();
+    }
+    if ( he == hipSuccess )
+        he = hipMemcpyAsync( &diff, dcount, sizeof( diff ), hipMemcpyDeviceToHost, h.stream );
+    if ( he == hipSuccess )
+        he = hipStreamSynchronize( h.stream );
+    *equal = diff == 0;
+    return he;
+}
+
+// The fused filter pass (forward passes, x H(k) inside the last one, inverse passes) in
+// each of its variants, with a Gaussian low-pass as the operator: the fastest one whose
+// output is bitwise equal to variant 0's is kept in h.flt_var.  Layouts whose filter
+// chain does not fuse have nothing to tune.
+int tune_filter( fftm_mi_ffts_s &h, const void *real_in, void *real_out, void *ref, size_t rbytes,
+                 unsigned long long *dcount, hipEvent_t e0, hipEvent_t e1, int warmup, int iterations,
+                 const std::string &key, cache_map &rec )
+{
+    if ( h.h16 || ( h.flags & FFTM_MI_FLAG_OVERWRITE_INPUT ) )
+        return ok();
+    h.flt_var   = -1;
+    h.flt_ready = false;
+    int rc      = ensure_filter_chain( h );
+    if ( rc != FFTM_MI_SUCCESS || !h.flt_fused )
+        return rc;
+    fftm_mi_spectral_op op{};
+    op.kind  = FFTM_MI_OP_GAUSSIAN;
+    op.scale = 1.0;
+    op.sigma = 8.0;
+    auto run = [&]() {
+        return h.prec == FFTM_MI_F32 ? filter_run<float>( h, real_in, real_out, op, false )
+                                     : filter_run<double>( h, real_in, real_out, op, false );
+    };
+    double best_ms = 1e300;
+    int    best    = -1;
+    for ( int var : FILTER_CANDIDATES )
+    {
+        h.flt_var   = var;
+        h.flt_ready = false;
+        rc          = ensure_filter_chain( h );
+        if ( rc == FFTM_MI_SUCCESS )
+            rc = run();
+        if ( rc != FFTM_MI_SUCCESS )
+            break;
+        bool       valid = true;
+        hipError_t he    = tune_compare( h, ref, real_out, rbytes, var == 0, dcount, &valid );
+        std::vector<float> ms;
+        for ( int w = 0; w < warmup && he == hipSuccess && rc == 0; ++w )
+            rc = run();
+        for ( int it = 0; it < iterations && he == hipSuccess && rc == 0; ++it )
+        {
+            float t = 0.f;
+            he      = hipEventRecord( e0, h.stream );
+            if ( he == hipSuccess )
+                rc = run();
+            if ( he == hipSuccess && rc == 0 )
+                he = hipEventRecord( e1, h.stream );
+            if ( he == hipSuccess && rc == 0 )
+                he = hipEventSynchronize( e1 );
+            if ( he == hipSuccess && rc == 0 )
+                he = hipEventElapsedTime( &t, e0, e1 );
+            ms.push_back( t );
+        }
+        if ( he != hipSuccess )
+            rc = fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_ffts_autotune: " ) + hipGetErrorString( he ) );
+        if ( rc != 0 )
+            break;
+        std::sort( ms.begin(), ms.end() );
+        const double med = ms.size() % 2 ? ms[ms.size() / 2] : 0.5 * ( ms[ms.size() / 2 - 1] + ms[ms.size() / 2] );
+        char         pfx[32];
+        std::snprintf( pfx, sizeof( pfx ), "_CANDIDATE_%03d_", var );
+        std::ostringstream mo;
+        mo << std::setprecision( 9 ) << med;
+        rec[key + pfx + "VALID"]     = valid ? "1" : "0";
+        rec[key + pfx + "MEDIAN_MS"] = valid ? mo.str() : "nan";
+        if ( valid && med < best_ms )
+        {
+            best_ms = med;
+            best    = var;
+        }
+    }
+    h.flt_var   = rc == 0 ? best : -1;
+    h.flt_ready = false;
+    if ( rc == 0 && best >= 0 )
+    {
+        rec[key] = std::to_string( best );
+        std::ostringstream mo;
+        mo << std::setprecision( 9 ) << best_ms;
+        rec[key + "_WINNER_MEDIAN_MS"] = mo.str();
+    }
+    return rc;
+}
+
 } // namespace
 
 extern "C" {
@@ -1907,13 +2037,20 @@ int fftm_mi_ffts_load_tuning( fftm_mi_ffts h, const char *cache_path, int32_t *a
             if ( it == m.end() )
                 continue;
             const int var = std::atoi( it->second.c_str() );
-            if ( var >= 0 && var < NUM_VARIANTS )
+            if ( variant_number_ok( var ) && ( !v[i].iom || mixed_variant_ok( var ) ) )
             {
                 v[i].var = var;
                 ++n;
             }
         }
     }
+    auto it = m.find( prob + "_FILTER" );
+    if ( it != m.end() && filter_variant_ok( std::atoi( it->second.c_str() ) ) )
+    {
+        h->flt_var = std::atoi( it->second.c_str() );
+        ++n;
+    }
+    h->flt_ready = false;
     if ( applied )
         *applied = n;
     return ok();
@@ -1979,9 +2116,13 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             std::vector<pass_rt> one( 1, cur );
             double               best_ms = 1e300;
             int                  best    = cur.var;
+            // tile shapes 0..nvar-1, each in both tile orders (plane and any-radix passes
+            // have one shape and their own order)
             const int            nvar    = ( cur.plane || cur.generic ) ? 1 : fixed_variants( cur.N );
-            for ( int var = 0; var < nvar && rc == 0; ++var )
+            const int            ncand   = ( cur.plane || cur.generic ) ? 1 : 2 * nvar;
+            for ( int cand = 0; cand < ncand && rc == 0; ++cand )
             {
+                const int var = cand < nvar ? cand : ( cand - nvar ) | VARIANT_RR;
                 if ( cur.iom && !mixed_variant_ok( var ) )
                     continue;
                 one[0].var = var;
@@ -1997,30 +2138,7 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
                 // every copy, clear and comparison is ordered on the handle's stream (which
                 // may be non-blocking with respect to the null stream)
                 bool       valid = true;
-                hipError_t he    = hipSuccess;
-                if ( var == 0 )
-                {
-                    he = hipMemcpyAsync( ref, dstbuf, dbytes, hipMemcpyDeviceToDevice, h->stream );
-                    if ( he == hipSuccess )
-                        he = hipStreamSynchronize( h->stream );
-                }
-                else
-                {
-                    unsigned long long diff = 0;
-                    he = hipMemsetAsync( dcount, 0, sizeof( unsigned long long ), h->stream );
-                    if ( he == hipSuccess )
-                    {
-                        hipLaunchKernelGGL( k_count_diff, dim3( 1024 ), dim3( 256 ), 0, h->stream,
-                                            static_cast<const uint32_t *>( ref ), static_cast<const uint32_t *>( dstbuf ),
-                                            (int64_t)( dbytes / 4 ), dcount );
-                        he = hipGetLastError();
-                    }
-                    if ( he == hipSuccess )
-                        he = hipMemcpyAsync( &diff, dcount, sizeof( diff ), hipMemcpyDeviceToHost, h->stream );
-                    if ( he == hipSuccess )
-                        he = hipStreamSynchronize( h->stream );
-                    valid = diff == 0;
-                }
+                hipError_t he    = tune_compare( *h, ref, dstbuf, dbytes, var == 0, dcount, &valid );
                 if ( he != hipSuccess )
                 {
                     rc = fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_ffts_autotune: " ) + hipGetErrorString( he ) );
@@ -2074,6 +2192,8 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             }
         }
     }
+    if ( rc == 0 )
+        rc = tune_filter( *h, real_in, real_out, ref, rbytes, dcount, e0, e1, warmup, iterations, prob + "_FILTER", rec );
     cleanup();
     if ( rc != 0 )
         return rc;

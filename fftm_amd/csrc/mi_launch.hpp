@@ -52,7 +52,13 @@ inline bool mixed_supported( int64_t N )
 }
 inline bool mixed_variant_ok( int var )
 {
-    return var == 0 || var == 6 || var == 7;
+    const int v = var & ~VARIANT_RR;
+    return v == 0 || v == 6 || v == 7;
+}
+// a variant number: a tile shape 0..NUM_VARIANTS-1, plus VARIANT_RR for the tile order
+inline bool variant_number_ok( int var )
+{
+    return var >= 0 && ( var & ~VARIANT_RR ) < NUM_VARIANTS;
 }
 hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, int iom,
                                   hipStream_t s );

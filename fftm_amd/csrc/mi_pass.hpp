@@ -300,6 +300,12 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 //   7: the tile of 0 exchanged in two rounds, registers capped for 4 waves per SIMD
 //      (some spilling; wins on the longest contiguous R2C lines)
 constexpr int NUM_VARIANTS = 8;
+// Bit 3 of a pass's variant number selects the tile order, a runtime choice: set, tile t
+// runs as workgroup t, so consecutive tiles go to different XCDs; clear, each XCD takes a
+// contiguous range of tiles (xcd_tile), so neighbouring tiles share one L2.  The host
+// passes the bit to the kernel as bit 2 of pass_params::vec.
+constexpr int VARIANT_RR = 8;
+constexpr int VEC_RR     = 4;
 
 constexpr bool variant_split( int var ) { return var >= 6 || var == 3; }
 constexpr int  variant_shape( int var ) { return var >= 6 ? 0 : ( var == 3 ? 4 : var ); }
@@ -1072,7 +1078,7 @@ struct pass_kernel
         const int tid = threadIdx.x;
         x.c          = LOADB ? tid % TW : tid / TPC;
         x.t          = LOADB ? tid / TW : tid % TPC;
-        x.tile       = xcd_tile( blockIdx.x, gridDim.x );
+        x.tile       = ( p.vec & VEC_RR ) ? (int64_t)blockIdx.x : xcd_tile( blockIdx.x, gridDim.x );
         const int64_t col = ( x.tile * TW + x.c ) * CPT;
         x.va         = col < p.ncols;
         x.vb         = CPT == 2 && col + 1 < p.ncols;

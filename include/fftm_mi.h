@@ -173,16 +173,17 @@ int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *b
  * passed; the pass reads and writes exactly what it does inside that call.  Used to
  * time individual kernels; it has no counterpart in the reference. */
 int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
-/* Tile-shape variant (0..7) of one pass's kernel: 0 default, 1 half the columns per
- * workgroup, 2 twice the threads per column, 3 the tile of 4 exchanged through LDS one
- * component at a time, 4 twice the columns (256-byte row segments, up to 160 KB of
- * LDS), 5 twice the columns and the threads per column,
- * 6 the tile of 0 exchanged through LDS one component at a time (half the LDS per
- * workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Results are bitwise
+/* Variant number (0..15) of one pass's kernel: a tile shape 0..7, plus 8 to hand the
+ * tiles to the XCDs round-robin instead of in one contiguous range per XCD.  Shapes:
+ * 0 default, 1 half the columns per workgroup, 2 twice the threads per column, 3 the
+ * tile of 4 exchanged through LDS one component at a time, 4 twice the columns
+ * (256-byte row segments, up to 160 KB of LDS), 5 twice the columns and the threads
+ * per column, 6 the tile of 0 exchanged through LDS one component at a time (half the
+ * LDS per workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Results are bitwise
  * identical across variants; only the speed differs.  Set by the tuner or by hand.
- * FP16-storage passes (FFTM_MI_F32_H16) have variants 0, 6 and 7.  Lengths compiled
- * with the default shape only (below 64, and the 3/5/7-smooth lengths) and plane or
- * any-radix passes run the default shape whatever the variant. */
+ * FP16-storage passes (FFTM_MI_F32_H16) have shapes 0, 6 and 7.  Lengths compiled
+ * with the default shape only (below 64, and the 3/5/7-smooth lengths) run the default
+ * shape whatever the variant; plane and any-radix passes also keep their own tile order. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
 int fftm_mi_ffts_get_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t *variant );
 /* Algorithmic HBM bytes one pass moves (reads + writes of its input and output). */
