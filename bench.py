@@ -61,6 +61,19 @@ def _traffic_summary(kernel_key):
         return None
 
 
+TUNING_CACHE = None  # --tuning-cache PATH
+
+
+def tune_handle(f, warmup, iterations):
+    """Tune f's passes, or load them from TUNING_CACHE when it already holds this problem
+    (the reference's KEY=VALUE cache: runs that must see the same variants share it)."""
+    if TUNING_CACHE is None:
+        return f.autotune(warmup=warmup, iterations=iterations)
+    if os.path.exists(TUNING_CACHE) and f.load_tuning(TUNING_CACHE) > 0:
+        return f.pass_variants()
+    return f.autotune(warmup=warmup, iterations=iterations, cache_path=TUNING_CACHE)
+
+
 def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None):
     """Time the forward + backward pair.  check: True (FP64 CPU oracle on the same field),
     "gpu64" (this library's FP64 transform of the same field as the reference, for sizes
@@ -81,8 +94,9 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
     h = f._handle
     L = _lib.lib()
     # plan-time tuning, outside the timed region (the reference's autotune protocol:
-    # every tile-shape variant of every pass timed, fastest bitwise-valid one kept)
-    variants = f.autotune(warmup=2, iterations=7) if tune else f.pass_variants()
+    # every variant of every pass timed, fastest bitwise-valid one kept); with
+    # --tuning-cache, a cache that already holds this problem is applied instead
+    variants = tune_handle(f, warmup=2, iterations=7) if tune else f.pass_variants()
 
     # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor), resident in HBM
     import oracle
@@ -239,7 +253,7 @@ def run_poisson(n, steps, warmup, seed=11):
     f.init(n, n, n, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", sync=False))
     stream = torch.cuda.current_stream()
     f.set_stream(stream)
-    f.autotune(warmup=2, iterations=5)
+    tune_handle(f, warmup=2, iterations=5)
     out = torch.empty_like(f32)
     X = f.complex_array()
     # the operator is the reference functor's H = -scale/|k|^2 (it solves Lap u = f);
@@ -333,7 +347,7 @@ def run_gaussian_4d(steps, warmup):
         f = fftm_amd.ffts("float32", strategy_4d=fftm_amd.strategy_4d_pencil_pencil())
         f.init(*dims, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", sync=False))
         f.set_stream(torch.cuda.current_stream())
-        f.autotune(warmup=1, iterations=3)
+        tune_handle(f, warmup=1, iterations=3)
         # real layout [w][z][y][x]
         ar = torch.arange(n, device="cuda", dtype=torch.float32) * (2 * math.pi / n)
         waves = torch.zeros((n, n, n, n), device="cuda")
@@ -428,12 +442,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-4d", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes (skip plan-time tuning)")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="autotune cache file: applied where it holds the problem, else written by the tuner")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 256^3, FP16-storage and Poisson secondary measurements")
     ap.add_argument("--big", action="store_true", help="also run the 1024^3 FP32 vs FP16-storage comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy 5 times (counter calibration)")
     args = ap.parse_args()
+    global TUNING_CACHE
+    TUNING_CACHE = args.tuning_cache
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (the transform runs on one device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
