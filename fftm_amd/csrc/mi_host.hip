@@ -276,13 +276,19 @@ hipError_t launch_plane_one( const pass_rt &r, const tables &tb, const void *src
     p.ci0     = r.ci0;
     p.ci1     = r.ci1;
     p.wmode   = r.wmode;
-    p.pad_    = 0;
+    p.order   = ( r.var & VARIANT_RR ) ? 1 : 0;
     p.scale   = (T)r.scale;
     const bool fwd = r.kind == KIND_R2C;
+    // shape 1: the register-resident kernel, where the B axis is contiguous in both
+    // arrays and the real rows 2q, 2q+1 can move as aligned pairs; otherwise shape 0
+    const void *real   = fwd ? src : static_cast<const void *>( dst );
+    const bool  rx     = ( r.var & ~VARIANT_RR ) == 1 && r.N == r.NB && r.rb == 1 && r.cb == 1 && r.ra % 2 == 0 &&
+                    ( r.pd0 <= 1 || r.ri0 % 2 == 0 ) && ( r.nplanes <= r.pd0 || r.ri1 % 2 == 0 ) &&
+                    ( reinterpret_cast<uintptr_t>( real ) % ( 2 * sizeof( T ) ) ) == 0;
     if constexpr ( std::is_same<T, float>::value )
-        return launch_plane_f32( p, (int)r.N, (int)r.NB, fwd, s );
+        return rx ? launch_plane_rx_f32( p, (int)r.N, fwd, s ) : launch_plane_f32( p, (int)r.N, (int)r.NB, fwd, s );
     else
-        return launch_plane_f64( p, (int)r.N, (int)r.NB, fwd, s );
+        return rx ? launch_plane_rx_f64( p, (int)r.N, fwd, s ) : launch_plane_f64( p, (int)r.N, (int)r.NB, fwd, s );
 }
 
 // Paired accesses (two consecutive elements per load / store) need unit element stride,
@@ -806,6 +812,9 @@ void fuse_plane( const ffts_layout &L, int32_t layout, int prec, std::vector<pas
         return;
     pass_rt p;
     p.plane   = true;
+    // shape 1, the register-resident kernel, where the layout allows it (it falls back to
+    // the LDS kernel elsewhere); 16-point planes are as fast on the LDS kernel
+    p.var     = na >= 32 ? 1 : 0;
     p.N       = na;
     p.NB      = nb;
     p.ra      = L.sr[A];
@@ -2116,10 +2125,11 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             std::vector<pass_rt> one( 1, cur );
             double               best_ms = 1e300;
             int                  best    = cur.var;
-            // tile shapes 0..nvar-1, each in both tile orders (plane and any-radix passes
-            // have one shape and their own order)
-            const int            nvar    = ( cur.plane || cur.generic ) ? 1 : fixed_variants( cur.N );
-            const int            ncand   = ( cur.plane || cur.generic ) ? 1 : 2 * nvar;
+            // tile shapes 0..nvar-1, each in both tile orders
+            // (plane passes: shape 0 the LDS kernel, 1 the register kernel; any-radix
+            // passes have one shape and their own order)
+            const int            nvar    = cur.generic ? 1 : ( cur.plane ? 2 : fixed_variants( cur.N ) );
+            const int            ncand   = cur.generic ? 1 : 2 * nvar;
             for ( int cand = 0; cand < ncand && rc == 0; ++cand )
             {
                 const int var = cand < nvar ? cand : ( cand - nvar ) | VARIANT_RR;

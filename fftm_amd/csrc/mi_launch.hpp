@@ -43,6 +43,10 @@ hipError_t launch_pass_f64( const pass_params<double> &p, int N, int kind, bool 
 bool       plane_supported( int64_t na, int64_t nb, int prec );
 hipError_t launch_plane_f32( const plane_params<float> &p, int na, int nb, bool fwd, hipStream_t s );
 hipError_t launch_plane_f64( const plane_params<double> &p, int na, int nb, bool fwd, hipStream_t s );
+// register-resident plane kernel (k_plane_rx): square planes 16..128 whose B axis is
+// contiguous in both arrays (rb = cb = 1)
+hipError_t launch_plane_rx_f32( const plane_params<float> &p, int n, bool fwd, hipStream_t s );
+hipError_t launch_plane_rx_f64( const plane_params<double> &p, int n, bool fwd, hipStream_t s );
 
 // FP16-stored spectra with FP32 arithmetic (iom: bit 0 FP16 input, bit 1 FP16 output);
 // lengths 128..2048, tile-shape variants 0, 6, 7

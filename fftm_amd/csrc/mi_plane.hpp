@@ -32,7 +32,7 @@ struct plane_params
     int64_t       nplanes, d0;      // planes; batch dim 0 extent (dim 1 = rest)
     int64_t       ri0, ri1, ci0, ci1; // real / complex strides of the two batch dims
     int32_t       wmode;            // real-axis weights, as pass_params::wmode
-    int32_t       pad_;
+    int32_t       order;            // 1: tile t runs as workgroup t (round-robin over XCDs)
     T             scale;
 };
 
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
     C            *twa   = lds + G::TWOFF;
     C            *twb   = twa + NA;
     const int     tid   = threadIdx.x;
-    const int64_t first = (int64_t)xcd_tile( blockIdx.x, gridDim.x ) * PPW;
+    const int64_t first = ( p.order ? (int64_t)blockIdx.x : xcd_tile( blockIdx.x, gridDim.x ) ) * PPW;
     int64_t       ro[PPW], co[PPW];
     bool          pv[PPW];
 #pragma unroll
@@ -347,6 +347,234 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
                 dst[off + p.rb]   = v.y * p.scale;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plane pass, register variant, for planes whose B axis is contiguous in both arrays
+// (rb = cb = 1: the w-x planes of the 4D pencil layout).  Each 1D transform keeps its
+// butterflies in registers: stage 0 takes its inputs straight from HBM (or, for the
+// second axis, from the first axis's result in LDS), the last stage hands its outputs
+// straight to HBM (or to LDS), and consecutive stages meet in one LDS exchange.  Per
+// plane that is about six LDS sweeps instead of k_plane's fourteen.  The arithmetic
+// (stages, twiddles, pair separation, Hermitian extension, scaling) is k_plane's, so
+// the two kernels give bitwise equal results and the tuner can choose either.
+// ---------------------------------------------------------------------------
+template <int N, int CS>
+struct plane_rx_geom
+{
+    static constexpr int NAH = N / 2 + 1;
+    static constexpr int NP  = N / 2;
+    // planes per workgroup and threads: small planes share a workgroup
+    static constexpr int PPW = N <= 16 ? 8 : ( N <= 32 ? 4 : ( N <= 64 ? 2 : 1 ) );
+    static constexpr int NT  = N >= 128 ? 512 : 256;
+    static constexpr int PX  = N + 1;       // row lines (A axis): odd pitch
+    static constexpr int XSH = 3;           // column lines (B axis): one pad every 8
+    static constexpr int PC  = N + ( N >> XSH );
+    static constexpr int ROWE  = PPW * NP * PX;
+    static constexpr int COLE  = PPW * NAH * PC;
+    static constexpr int ELEMS = ROWE > COLE ? ROWE : COLE;
+    static constexpr int TOTAL = ELEMS + 2 * N; // + twiddle tables (A, then B)
+    static constexpr int BYTES = TOTAL * CS;
+};
+
+// Stockham stages of NL lines of length N: stage 0 reads ld(line, e), the last stage
+// calls st(line, e, value), and stages in between exchange through LDS at
+// xb[line * XP + e + (e >> XSH)].  LF: consecutive threads take consecutive lines, else
+// consecutive butterflies of one line.  ST_LDS: st writes LDS that the stages may have
+// read, so a barrier precedes it.
+template <typename T, int N, int NL, bool INV, int NT, bool LF, int XP, int XSH, bool ST_LDS>
+struct rx_lines
+{
+    using C                   = cpx<T>;
+    static constexpr rplan RP = make_rplan( N );
+    __device__ static __forceinline__ int at( int line, int e ) { return line * XP + ( XSH ? e + ( e >> XSH ) : e ); }
+
+    template <int S, typename LD, typename ST>
+    __device__ static __forceinline__ void run( C *xb, const C *tw, const LD &ld, const ST &st )
+    {
+        constexpr int R     = RP.r[S];
+        constexpr int NS    = rplan_span( RP, S );
+        constexpr int NBF   = N / R;
+        constexpr int ITEMS = NL * NBF;
+        constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
+        constexpr int TST   = N / ( NS * R );
+        const int     tid   = threadIdx.x;
+        C             v[PER][R];
+#pragma unroll
+        for ( int i = 0; i < PER; ++i )
+        {
+            const int item = tid + i * NT;
+            if ( ITEMS % NT == 0 || item < ITEMS )
+            {
+                const int line = LF ? item % NL : item / NBF;
+                const int j    = LF ? item / NL : item % NBF;
+#pragma unroll
+                for ( int r = 0; r < R; ++r )
+                {
+                    if constexpr ( S == 0 )
+                        v[i][r] = ld( line, j + r * NBF );
+                    else
+                        v[i][r] = xb[at( line, j + r * NBF )];
+                }
+                const int k = j % NS;
+                if constexpr ( NS > 1 )
+                {
+#pragma unroll
+                    for ( int r = 1; r < R; ++r )
+                        v[i][r] = twmul<INV>( v[i][r], tw[r * k * TST] );
+                }
+                dft_fixed<R, INV>( v[i], tw, N );
+            }
+        }
+        if constexpr ( S + 1 < RP.n )
+        {
+            __syncthreads(); // every read of xb (this stage, or the previous phase) is done
+#pragma unroll
+            for ( int i = 0; i < PER; ++i )
+            {
+                const int item = tid + i * NT;
+                if ( ITEMS % NT == 0 || item < ITEMS )
+                {
+                    const int line = LF ? item % NL : item / NBF;
+                    const int j    = LF ? item / NL : item % NBF;
+                    const int d    = ( j / NS ) * NS * R + ( j % NS );
+#pragma unroll
+                    for ( int m = 0; m < R; ++m )
+                        xb[at( line, d + m * NS )] = v[i][m];
+                }
+            }
+            __syncthreads();
+            run<S + 1>( xb, tw, ld, st );
+        }
+        else
+        {
+            if constexpr ( ST_LDS )
+                __syncthreads();
+#pragma unroll
+            for ( int i = 0; i < PER; ++i )
+            {
+                const int item = tid + i * NT;
+                if ( ITEMS % NT == 0 || item < ITEMS )
+                {
+                    const int line = LF ? item % NL : item / NBF;
+                    const int j    = LF ? item / NL : item % NBF;
+                    const int d    = ( j / NS ) * NS * R + ( j % NS );
+#pragma unroll
+                    for ( int m = 0; m < R; ++m )
+                        st( line, d + m * NS, v[i][m] );
+                }
+            }
+        }
+    }
+};
+
+// FWD: real -> spectrum (R2C along A, C2C forward along B); else the inverse.  Square
+// planes (NA = NB = N); the host launches it only for rb = cb = 1 with even real-array
+// offsets and an 8-byte aligned real array (its rows 2q, 2q+1 move as one 8-byte pair).
+template <typename T, int N, bool FWD>
+__global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT ) )
+    k_plane_rx( const plane_params<T> p )
+{
+    using C               = cpx<T>;
+    using G               = plane_rx_geom<N, (int)sizeof( C )>;
+    using V2              = typename pair_of<T>::type;
+    constexpr int NT      = G::NT;
+    constexpr int PPW     = G::PPW;
+    constexpr int NAH     = G::NAH;
+    constexpr int NP      = G::NP;
+    static_assert( G::BYTES <= plane_lds_limit(), "plane does not fit in LDS" );
+    using ROWS = rx_lines<T, N, PPW * NP, !FWD, NT, true, G::PX, 0, FWD>;
+    using COLS = rx_lines<T, N, PPW * NAH, !FWD, NT, false, G::PC, G::XSH, !FWD>;
+    __shared__ C  lds[G::TOTAL];
+    C            *twa   = lds + G::ELEMS;
+    C            *twb   = twa + N;
+    const int     tid   = threadIdx.x;
+    const int64_t first = ( p.order ? (int64_t)blockIdx.x : xcd_tile( blockIdx.x, gridDim.x ) ) * PPW;
+    int64_t       ro[PPW], co[PPW];
+    bool          pv[PPW];
+#pragma unroll
+    for ( int g = 0; g < PPW; ++g )
+    {
+        pv[g] = first + g < p.nplanes;
+        plane_offsets( p, pv[g] ? first + g : 0, ro[g], co[g] );
+    }
+    // twiddle tables into LDS; stage 0 of every transform (radix 2, 4, 8 or 16) needs
+    // none, and later stages read them after an exchange barrier
+    for ( int e = tid; e < 2 * N; e += NT )
+        twa[e] = e < N ? p.twa[e] : p.twb[e - N];
+
+    if constexpr ( FWD )
+    {
+        // rows: Z[q][n] = (x[2q][n], x[2q+1][n]) straight from HBM, Z to LDS in natural order
+        auto ld_row = [&]( int line, int n ) -> C {
+            const int g = line / NP, q = line % NP;
+            if ( !pv[g] )
+                return mk<T>( 0, 0 );
+            const V2 w = *reinterpret_cast<const V2 *>( static_cast<const T *>( p.in ) + ro[g] + (int64_t)n * p.ra + 2 * q );
+            return mk<T>( w.x, w.y );
+        };
+        auto st_row = [&]( int line, int n, const C &v ) { lds[ROWS::at( line, n )] = v; };
+        ROWS::template run<0>( lds, twa, ld_row, st_row );
+        __syncthreads();
+        // columns: S[b][k] separated from Z rows b / 2 (k and its mirror) on the way in
+        auto ld_col = [&]( int col, int b ) -> C {
+            const int g = col / NAH, k = col % NAH, line = g * NP + b / 2;
+            const C   zk = lds[ROWS::at( line, k )];
+            const C   zm = lds[ROWS::at( line, k == 0 ? 0 : N - k )];
+            const T   h  = ( p.wmode == 1 && k != 0 && 2 * k != N ) ? (T)1 : (T)0.5;
+            return ( b & 1 ) ? mk<T>( h * ( zk.y + zm.y ), h * ( zm.x - zk.x ) )
+                             : mk<T>( h * ( zk.x + zm.x ), h * ( zk.y - zm.y ) );
+        };
+        auto st_col = [&]( int col, int b, const C &v ) {
+            const int g = col / NAH, k = col % NAH;
+            if ( pv[g] )
+                static_cast<C *>( p.out )[co[g] + (int64_t)k * p.ca + b] = scl( v, p.scale );
+        };
+        COLS::template run<0>( lds, twb, ld_col, st_col );
+    }
+    else
+    {
+        // columns, inverse: S straight from HBM, S to LDS in natural order
+        auto ld_col = [&]( int col, int b ) -> C {
+            const int g = col / NAH, k = col % NAH;
+            if ( !pv[g] )
+                return mk<T>( 0, 0 );
+            return static_cast<const C *>( p.in )[co[g] + (int64_t)k * p.ca + b];
+        };
+        auto st_col = [&]( int col, int b, const C &v ) { lds[COLS::at( col, b )] = v; };
+        COLS::template run<0>( lds, twb, ld_col, st_col );
+        __syncthreads();
+        // rows, inverse: Z[q][n] = A~[n] + i B~[n] (Hermitian extension) on the way in
+        auto ld_row = [&]( int line, int n ) -> C {
+            const int g = line / NP, q = line % NP;
+            const int nn = n <= N / 2 ? n : N - n;
+            C         av = lds[COLS::at( g * NAH + nn, 2 * q )];
+            C         bv = lds[COLS::at( g * NAH + nn, 2 * q + 1 )];
+            if ( n > N / 2 )
+            {
+                av.y = -av.y;
+                bv.y = -bv.y;
+            }
+            if ( n == 0 || 2 * n == N )
+            {
+                av.y = 0;
+                bv.y = 0;
+            }
+            else if ( p.wmode == 2 )
+            {
+                av = scl( av, (T)0.5 );
+                bv = scl( bv, (T)0.5 );
+            }
+            return mk<T>( av.x - bv.y, av.y + bv.x );
+        };
+        auto st_row = [&]( int line, int n, const C &v ) {
+            const int g = line / NP, q = line % NP;
+            if ( pv[g] )
+                *reinterpret_cast<V2 *>( static_cast<T *>( p.out ) + ro[g] + (int64_t)n * p.ra + 2 * q ) =
+                    V2{ v.x * p.scale, v.y * p.scale };
+        };
+        ROWS::template run<0>( lds, twa, ld_row, st_row );
     }
 }
 

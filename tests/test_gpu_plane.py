@@ -101,3 +101,50 @@ def test_plane_pass_adjoints(gpu, kind, dims):
     a = float(torch.sum(CX * v))
     b = float(np.sum(from_gpu(X).real * from_gpu(Ctv).real + from_gpu(X).imag * from_gpu(Ctv).imag))
     assert abs(a - b) <= 1e-11 * abs(a)
+
+
+# pencil planes (B = x contiguous in both arrays): shape 1 runs the register-resident
+# plane kernel; plane counts that leave the last workgroup partly empty included
+RX_CASES = [("float32", (16, 15, 35, 16)), ("float32", (32, 32, 32, 32)), ("float32", (64, 27, 21, 64)),
+            ("float32", (128, 8, 64, 128)), ("float64", (64, 27, 21, 64)), ("float64", (128, 8, 64, 128))]
+
+
+@pytest.mark.parametrize("real,dims", RX_CASES)
+def test_plane_register_kernel_bitwise(gpu, real, dims):
+    """Plane-pass variants 1 (register-resident kernel), 8 and 9 (round-robin tile order)
+    reproduce variant 0 bitwise: forward, backward and both adjoints (weight modes 1, 2)."""
+    import torch
+    from fftm_amd import _lib
+    L = _lib.lib()
+    f = _make("pencil", dims, real)
+    nf, nb = f.num_passes()
+    print(f"case plane-rx pencil {dims} {real}", flush=True)
+    assert nf == 3 and nb == 3, "plane fusion expected"
+    dt = np.float32 if real == "float32" else np.float64
+    ct = np.complex64 if real == "float32" else np.complex128
+    g = np.random.default_rng(11)
+    x = g.uniform(-1, 1, f.real_shape()).astype(dt)
+    u = to_gpu(x)
+    S = to_gpu((g.standard_normal(f.complex_shape()) + 1j * g.standard_normal(f.complex_shape())).astype(ct))
+
+    def run():
+        X = f.complex_array()
+        f.forward(u, X)
+        y = f.real_array()
+        f.backward(S, y)
+        A = f.real_array()
+        f.r2c_adjoint(S, A)
+        B = f.complex_array()
+        f.c2r_adjoint(u, B)
+        return X, y, A, B
+
+    ref = run()
+    check_close(from_gpu(ref[0]), _oracle_fwd("pencil", x.astype(np.float64), dims), dt, f"plane-rx fwd {dims}")
+    for var in (1, 8, 9):
+        for bw, n in ((0, nf), (1, nb)):
+            for i in range(n):
+                _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
+        out = run()
+        for a, b, name in zip(ref, out, ("forward", "backward", "r2c_adjoint", "c2r_adjoint")):
+            assert torch.equal(a, b), f"variant {var}: {name} differs from variant 0"
+    f.close()
