@@ -279,12 +279,16 @@ hipError_t launch_plane_one( const pass_rt &r, const tables &tb, const void *src
     p.order   = ( r.var & VARIANT_RR ) ? 1 : 0;
     p.scale   = (T)r.scale;
     const bool fwd = r.kind == KIND_R2C;
-    // shape 1: the register-resident kernel, where the B axis is contiguous in both
-    // arrays and the real rows 2q, 2q+1 can move as aligned pairs; otherwise shape 0
-    const void *real   = fwd ? src : static_cast<const void *>( dst );
-    const bool  rx     = ( r.var & ~VARIANT_RR ) == 1 && r.N == r.NB && r.rb == 1 && r.cb == 1 && r.ra % 2 == 0 &&
-                    ( r.pd0 <= 1 || r.ri0 % 2 == 0 ) && ( r.nplanes <= r.pd0 || r.ri1 % 2 == 0 ) &&
-                    ( reinterpret_cast<uintptr_t>( real ) % ( 2 * sizeof( T ) ) ) == 0;
+    // shape 1: the register-resident kernel (square planes); its real rows 2q, 2q+1 move
+    // as one pair where B is contiguous, every row start is even and the array aligned
+    const void *real = fwd ? src : static_cast<const void *>( dst );
+    const bool  rx   = ( r.var & ~VARIANT_RR ) == 1 && r.N == r.NB;
+    p.pair           = ( r.rb == 1 && r.ra % 2 == 0 && ( r.pd0 <= 1 || r.ri0 % 2 == 0 ) &&
+               ( r.nplanes <= r.pd0 || r.ri1 % 2 == 0 ) &&
+               ( reinterpret_cast<uintptr_t>( real ) % ( 2 * sizeof( T ) ) ) == 0 )
+                         ? 1
+                         : 0;
+    p.pad_ = 0;
     if constexpr ( std::is_same<T, float>::value )
         return rx ? launch_plane_rx_f32( p, (int)r.N, fwd, s ) : launch_plane_f32( p, (int)r.N, (int)r.NB, fwd, s );
     else
@@ -812,13 +816,13 @@ void fuse_plane( const ffts_layout &L, int32_t layout, int prec, std::vector<pas
         return;
     pass_rt p;
     p.plane   = true;
-    // shape 1, the register-resident kernel, where the layout allows it (it falls back to
-    // the LDS kernel elsewhere); 16-point planes are as fast on the LDS kernel
-    p.var     = na >= 32 ? 1 : 0;
     p.N       = na;
     p.NB      = nb;
     p.ra      = L.sr[A];
     p.rb      = L.sr[B];
+    // default shape: the register-resident kernel (1) from 32 points up, measured faster
+    // or level in every layout; the tuner times both
+    p.var     = na >= 32 ? 1 : 0;
     p.ca      = L.sc[A];
     p.cb      = L.sc[B];
     p.nplanes = nplanes;

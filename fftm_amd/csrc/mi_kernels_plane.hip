@@ -36,10 +36,16 @@ hipError_t launch_plane_rx_n( const plane_params<T> &p, bool fwd, hipStream_t s 
     if ( groups > 0x7fffffffLL )
         return hipErrorInvalidConfiguration;
     const dim3 grid( (unsigned)groups ), block( G::NT );
-    if ( fwd )
-        hipLaunchKernelGGL( ( k_plane_rx<T, N, true> ), grid, block, 0, s, p );
+    // real rows staged through LDS unless B is contiguous in the real array
+    const bool staged = p.rb != 1;
+    if ( fwd && staged )
+        hipLaunchKernelGGL( ( k_plane_rx<T, N, true, true> ), grid, block, 0, s, p );
+    else if ( fwd )
+        hipLaunchKernelGGL( ( k_plane_rx<T, N, true, false> ), grid, block, 0, s, p );
+    else if ( staged )
+        hipLaunchKernelGGL( ( k_plane_rx<T, N, false, true> ), grid, block, 0, s, p );
     else
-        hipLaunchKernelGGL( ( k_plane_rx<T, N, false> ), grid, block, 0, s, p );
+        hipLaunchKernelGGL( ( k_plane_rx<T, N, false, false> ), grid, block, 0, s, p );
     return hipGetLastError();
 }
 

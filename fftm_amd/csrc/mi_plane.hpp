@@ -33,6 +33,8 @@ struct plane_params
     int64_t       ri0, ri1, ci0, ci1; // real / complex strides of the two batch dims
     int32_t       wmode;            // real-axis weights, as pass_params::wmode
     int32_t       order;            // 1: tile t runs as workgroup t (round-robin over XCDs)
+    int32_t       pair;             // k_plane_rx: real rows 2q, 2q+1 move as one aligned pair
+    int32_t       pad_;
     T             scale;
 };
 
@@ -166,10 +168,12 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
         plane_offsets( p, pv[g] ? first + g : 0, ro[g], co[g] );
     }
     auto zi = []( int g, int q, int n ) { return g * EL + q * PZ + n + ( n >> ZSH ); }; // Z[q][n] of plane g
-
-    // twiddle tables into LDS (read once per workgroup, from L2)
-    for ( int e = tid; e < NA + NB; e += NT )
-        twa[e] = e < NA ? p.twa[e] : p.twb[e - NA];
+    // twiddle tables into LDS (read once per workgroup, from L2), after the plane's loads
+    // are issued so that their latency overlaps
+    auto copy_twiddles = [&]() {
+        for ( int e = tid; e < NA + NB; e += NT )
+            twa[e] = e < NA ? p.twa[e] : p.twb[e - NA];
+    };
 
     constexpr int ITZ = NP * NA;  // Z elements (pairs of reals) per plane
     constexpr int PZT = ( PPW * ITZ + NT - 1 ) / NT;
@@ -198,6 +202,7 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
                 ua[i] = ub[i] = (T)0;
             }
         }
+        copy_twiddles();
 #pragma unroll
         for ( int i = 0; i < PZT; ++i )
         {
@@ -276,6 +281,7 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
                 u[i]          = src[(int64_t)k * p.ca + (int64_t)bb * p.cb];
             }
         }
+        copy_twiddles();
 #pragma unroll
         for ( int i = 0; i < PST; ++i )
         {
@@ -351,14 +357,15 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
 }
 
 // ---------------------------------------------------------------------------
-// Plane pass, register variant, for planes whose B axis is contiguous in both arrays
-// (rb = cb = 1: the w-x planes of the 4D pencil layout).  Each 1D transform keeps its
+// Plane pass, register variant (square planes).  Each 1D transform keeps its
 // butterflies in registers: stage 0 takes its inputs straight from HBM (or, for the
 // second axis, from the first axis's result in LDS), the last stage hands its outputs
 // straight to HBM (or to LDS), and consecutive stages meet in one LDS exchange.  Per
-// plane that is about six LDS sweeps instead of k_plane's fourteen.  The arithmetic
-// (stages, twiddles, pair separation, Hermitian extension, scaling) is k_plane's, so
-// the two kernels give bitwise equal results and the tuner can choose either.
+// plane that is about six LDS sweeps instead of k_plane's fourteen.  Consecutive
+// threads walk whichever axis is contiguous in HBM (runtime choice per phase).  The
+// arithmetic (stages, twiddles, pair separation, Hermitian extension, scaling) is
+// k_plane's, so the two kernels give bitwise equal results and the tuner can choose
+// either.
 // ---------------------------------------------------------------------------
 template <int N, int CS>
 struct plane_rx_geom
@@ -368,11 +375,14 @@ struct plane_rx_geom
     // planes per workgroup and threads: small planes share a workgroup
     static constexpr int PPW = N <= 16 ? 8 : ( N <= 32 ? 4 : ( N <= 64 ? 2 : 1 ) );
     static constexpr int NT  = N >= 128 ? 512 : 256;
-    static constexpr int PX  = N + 1;       // row lines (A axis): odd pitch
-    static constexpr int XSH = 3;           // column lines (B axis): one pad every 8
+    static constexpr int XSH = 3;                    // one pad element every 8
+    static constexpr int PX  = N + ( N >> XSH ) + 1; // row lines (A axis): odd pitch
+    // column lines (B axis): an even pitch when consecutive threads take the butterflies
+    // of one column, an odd one when they take consecutive columns
     static constexpr int PC  = N + ( N >> XSH );
+    static constexpr int PCO = PC + 1;
     static constexpr int ROWE  = PPW * NP * PX;
-    static constexpr int COLE  = PPW * NAH * PC;
+    static constexpr int COLE  = PPW * NAH * PCO;
     static constexpr int ELEMS = ROWE > COLE ? ROWE : COLE;
     static constexpr int TOTAL = ELEMS + 2 * N; // + twiddle tables (A, then B)
     static constexpr int BYTES = TOTAL * CS;
@@ -380,18 +390,19 @@ struct plane_rx_geom
 
 // Stockham stages of NL lines of length N: stage 0 reads ld(line, e), the last stage
 // calls st(line, e, value), and stages in between exchange through LDS at
-// xb[line * XP + e + (e >> XSH)].  LF: consecutive threads take consecutive lines, else
-// consecutive butterflies of one line.  ST_LDS: st writes LDS that the stages may have
-// read, so a barrier precedes it.
-template <typename T, int N, int NL, bool INV, int NT, bool LF, int XP, int XSH, bool ST_LDS>
+// xb[line * xp + e + (e >> XSH)].  lf: consecutive threads take consecutive lines, else
+// consecutive butterflies of one line.  st_lds: st writes LDS that the stages may have
+// read, so a barrier precedes it.  hook runs once stage 0 has issued its loads.
+template <typename T, int N, int NL, bool INV, int NT, int XSH>
 struct rx_lines
 {
     using C                   = cpx<T>;
     static constexpr rplan RP = make_rplan( N );
-    __device__ static __forceinline__ int at( int line, int e ) { return line * XP + ( XSH ? e + ( e >> XSH ) : e ); }
+    __device__ static __forceinline__ int at( int line, int e, int xp ) { return line * xp + ( XSH ? e + ( e >> XSH ) : e ); }
 
-    template <int S, typename LD, typename ST>
-    __device__ static __forceinline__ void run( C *xb, const C *tw, const LD &ld, const ST &st )
+    template <int S, typename LD, typename ST, typename HK>
+    __device__ static __forceinline__ void run( C *xb, int xp, const C *tw, const LD &ld, const ST &st, bool lf,
+                                                bool st_lds, const HK &hook )
     {
         constexpr int R     = RP.r[S];
         constexpr int NS    = rplan_span( RP, S );
@@ -407,15 +418,15 @@ struct rx_lines
             const int item = tid + i * NT;
             if ( ITEMS % NT == 0 || item < ITEMS )
             {
-                const int line = LF ? item % NL : item / NBF;
-                const int j    = LF ? item / NL : item % NBF;
+                const int line = lf ? item % NL : item / NBF;
+                const int j    = lf ? item / NL : item % NBF;
 #pragma unroll
                 for ( int r = 0; r < R; ++r )
                 {
                     if constexpr ( S == 0 )
                         v[i][r] = ld( line, j + r * NBF );
                     else
-                        v[i][r] = xb[at( line, j + r * NBF )];
+                        v[i][r] = xb[at( line, j + r * NBF, xp )];
                 }
                 const int k = j % NS;
                 if constexpr ( NS > 1 )
@@ -427,6 +438,8 @@ struct rx_lines
                 dft_fixed<R, INV>( v[i], tw, N );
             }
         }
+        if constexpr ( S == 0 )
+            hook(); // after stage 0 has issued its loads, before the first barrier
         if constexpr ( S + 1 < RP.n )
         {
             __syncthreads(); // every read of xb (this stage, or the previous phase) is done
@@ -436,20 +449,20 @@ struct rx_lines
                 const int item = tid + i * NT;
                 if ( ITEMS % NT == 0 || item < ITEMS )
                 {
-                    const int line = LF ? item % NL : item / NBF;
-                    const int j    = LF ? item / NL : item % NBF;
+                    const int line = lf ? item % NL : item / NBF;
+                    const int j    = lf ? item / NL : item % NBF;
                     const int d    = ( j / NS ) * NS * R + ( j % NS );
 #pragma unroll
                     for ( int m = 0; m < R; ++m )
-                        xb[at( line, d + m * NS )] = v[i][m];
+                        xb[at( line, d + m * NS, xp )] = v[i][m];
                 }
             }
             __syncthreads();
-            run<S + 1>( xb, tw, ld, st );
+            run<S + 1>( xb, xp, tw, ld, st, lf, st_lds, hook );
         }
         else
         {
-            if constexpr ( ST_LDS )
+            if ( st_lds )
                 __syncthreads();
 #pragma unroll
             for ( int i = 0; i < PER; ++i )
@@ -457,8 +470,8 @@ struct rx_lines
                 const int item = tid + i * NT;
                 if ( ITEMS % NT == 0 || item < ITEMS )
                 {
-                    const int line = LF ? item % NL : item / NBF;
-                    const int j    = LF ? item / NL : item % NBF;
+                    const int line = lf ? item % NL : item / NBF;
+                    const int j    = lf ? item / NL : item % NBF;
                     const int d    = ( j / NS ) * NS * R + ( j % NS );
 #pragma unroll
                     for ( int m = 0; m < R; ++m )
@@ -470,9 +483,11 @@ struct rx_lines
 };
 
 // FWD: real -> spectrum (R2C along A, C2C forward along B); else the inverse.  Square
-// planes (NA = NB = N); the host launches it only for rb = cb = 1 with even real-array
-// offsets and an 8-byte aligned real array (its rows 2q, 2q+1 move as one 8-byte pair).
-template <typename T, int N, bool FWD>
+// planes (NA = NB = N).  STAGED (for planes whose B axis is not contiguous in the real
+// array): the real rows go through LDS with consecutive threads along A.  Otherwise the
+// threads walk the rows q, and p.pair (set by the host for rb = 1, even real-array
+// offsets and an aligned real array) moves rows 2q, 2q+1 as one pair of values.
+template <typename T, int N, bool FWD, bool STAGED>
 __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT ) )
     k_plane_rx( const plane_params<T> p )
 {
@@ -484,8 +499,18 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
     constexpr int NAH     = G::NAH;
     constexpr int NP      = G::NP;
     static_assert( G::BYTES <= plane_lds_limit(), "plane does not fit in LDS" );
-    using ROWS = rx_lines<T, N, PPW * NP, !FWD, NT, true, G::PX, 0, FWD>;
-    using COLS = rx_lines<T, N, PPW * NAH, !FWD, NT, false, G::PC, G::XSH, !FWD>;
+    using ROWS = rx_lines<T, N, PPW * NP, !FWD, NT, G::XSH>;
+    using COLS = rx_lines<T, N, PPW * NAH, !FWD, NT, G::XSH>;
+    constexpr int PX = G::PX;
+    // consecutive threads along the axis that is contiguous in HBM: the rows (q) when B
+    // is contiguous in the real array; else the real rows go through LDS with coalesced
+    // accesses along A (`staged`).  The columns (k) when the half axis is contiguous in
+    // the spectrum, else the butterflies of a column (b)
+    constexpr bool staged = STAGED;
+    const bool     lf_cols = p.ca == 1;
+    const int      pc      = lf_cols ? G::PCO : G::PC;
+    constexpr int ITZ  = PPW * NP * N; // staged real-row elements (pairs of reals)
+    constexpr int PZT  = ( ITZ + NT - 1 ) / NT;
     __shared__ C  lds[G::TOTAL];
     C            *twa   = lds + G::ELEMS;
     C            *twb   = twa + N;
@@ -499,29 +524,78 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
         pv[g] = first + g < p.nplanes;
         plane_offsets( p, pv[g] ? first + g : 0, ro[g], co[g] );
     }
-    // twiddle tables into LDS; stage 0 of every transform (radix 2, 4, 8 or 16) needs
-    // none, and later stages read them after an exchange barrier
-    for ( int e = tid; e < 2 * N; e += NT )
-        twa[e] = e < N ? p.twa[e] : p.twb[e - N];
-
+    // twiddle tables into LDS once the first HBM loads are issued; stage 0 of every
+    // transform (radix 2, 4, 8 or 16) needs none, and later stages read them after an
+    // exchange barrier
+    auto copy_twiddles = [&]() {
+        for ( int e = tid; e < 2 * N; e += NT )
+            twa[e] = e < N ? p.twa[e] : p.twb[e - N];
+    };
+    auto no_hook = []() {};
     if constexpr ( FWD )
     {
         // rows: Z[q][n] = (x[2q][n], x[2q+1][n]) straight from HBM, Z to LDS in natural order
+        if constexpr ( staged )
+        {
+            // Z[q][n] into the row layout, consecutive threads along n; loads in chunks
+            // of 8 per thread, all issued before the chunk's LDS writes
+            constexpr int CH = 8;
+#pragma unroll
+            for ( int i0 = 0; i0 < PZT; i0 += CH )
+            {
+                T ua[CH], ub[CH];
+#pragma unroll
+                for ( int c = 0; c < CH; ++c )
+                {
+                    // unconditional loads (an element past the end reads the last one, an
+                    // empty plane slot reads plane 0) so that none waits on a branch
+                    const int i = i0 + c < PZT ? i0 + c : PZT - 1, e0 = tid + i * NT;
+                    const int e = ( ITZ % NT == 0 || e0 < ITZ ) ? e0 : ITZ - 1;
+                    const int g = e / ( NP * N ), r = e % ( NP * N );
+                    const T  *src = static_cast<const T *>( p.in ) + ro[g] + (int64_t)( r % N ) * p.ra +
+                                   (int64_t)( 2 * ( r / N ) ) * p.rb;
+                    const T   a   = src[0];
+                    const T   b   = src[p.rb];
+                    ua[c]         = pv[g] ? a : (T)0;
+                    ub[c]         = pv[g] ? b : (T)0;
+                }
+                if ( i0 == 0 )
+                    copy_twiddles();
+#pragma unroll
+                for ( int c = 0; c < CH; ++c )
+                {
+                    const int i = i0 + c, e = tid + i * NT, g = e / ( NP * N ), r = e % ( NP * N );
+                    if ( i < PZT && ( ITZ % NT == 0 || e < ITZ ) )
+                        lds[ROWS::at( g * NP + r / N, r % N, PX )] = mk<T>( ua[c], ub[c] );
+                }
+            }
+            __syncthreads();
+        }
         auto ld_row = [&]( int line, int n ) -> C {
+            if constexpr ( staged )
+                return lds[ROWS::at( line, n, PX )];
             const int g = line / NP, q = line % NP;
             if ( !pv[g] )
                 return mk<T>( 0, 0 );
-            const V2 w = *reinterpret_cast<const V2 *>( static_cast<const T *>( p.in ) + ro[g] + (int64_t)n * p.ra + 2 * q );
-            return mk<T>( w.x, w.y );
+            const T *src = static_cast<const T *>( p.in ) + ro[g] + (int64_t)n * p.ra + (int64_t)( 2 * q ) * p.rb;
+            if ( p.pair )
+            {
+                const V2 w = *reinterpret_cast<const V2 *>( src );
+                return mk<T>( w.x, w.y );
+            }
+            return mk<T>( src[0], src[p.rb] );
         };
-        auto st_row = [&]( int line, int n, const C &v ) { lds[ROWS::at( line, n )] = v; };
-        ROWS::template run<0>( lds, twa, ld_row, st_row );
+        auto st_row = [&]( int line, int n, const C &v ) { lds[ROWS::at( line, n, PX )] = v; };
+        if constexpr ( staged )
+            ROWS::template run<0>( lds, PX, twa, ld_row, st_row, true, true, no_hook );
+        else
+            ROWS::template run<0>( lds, PX, twa, ld_row, st_row, true, true, copy_twiddles );
         __syncthreads();
         // columns: S[b][k] separated from Z rows b / 2 (k and its mirror) on the way in
         auto ld_col = [&]( int col, int b ) -> C {
             const int g = col / NAH, k = col % NAH, line = g * NP + b / 2;
-            const C   zk = lds[ROWS::at( line, k )];
-            const C   zm = lds[ROWS::at( line, k == 0 ? 0 : N - k )];
+            const C   zk = lds[ROWS::at( line, k, PX )];
+            const C   zm = lds[ROWS::at( line, k == 0 ? 0 : N - k, PX )];
             const T   h  = ( p.wmode == 1 && k != 0 && 2 * k != N ) ? (T)1 : (T)0.5;
             return ( b & 1 ) ? mk<T>( h * ( zk.y + zm.y ), h * ( zm.x - zk.x ) )
                              : mk<T>( h * ( zk.x + zm.x ), h * ( zk.y - zm.y ) );
@@ -529,9 +603,9 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
         auto st_col = [&]( int col, int b, const C &v ) {
             const int g = col / NAH, k = col % NAH;
             if ( pv[g] )
-                static_cast<C *>( p.out )[co[g] + (int64_t)k * p.ca + b] = scl( v, p.scale );
+                static_cast<C *>( p.out )[co[g] + (int64_t)k * p.ca + (int64_t)b * p.cb] = scl( v, p.scale );
         };
-        COLS::template run<0>( lds, twb, ld_col, st_col );
+        COLS::template run<0>( lds, pc, twb, ld_col, st_col, lf_cols, false, no_hook );
     }
     else
     {
@@ -540,17 +614,17 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
             const int g = col / NAH, k = col % NAH;
             if ( !pv[g] )
                 return mk<T>( 0, 0 );
-            return static_cast<const C *>( p.in )[co[g] + (int64_t)k * p.ca + b];
+            return static_cast<const C *>( p.in )[co[g] + (int64_t)k * p.ca + (int64_t)b * p.cb];
         };
-        auto st_col = [&]( int col, int b, const C &v ) { lds[COLS::at( col, b )] = v; };
-        COLS::template run<0>( lds, twb, ld_col, st_col );
+        auto st_col = [&]( int col, int b, const C &v ) { lds[COLS::at( col, b, pc )] = v; };
+        COLS::template run<0>( lds, pc, twb, ld_col, st_col, lf_cols, true, copy_twiddles );
         __syncthreads();
         // rows, inverse: Z[q][n] = A~[n] + i B~[n] (Hermitian extension) on the way in
         auto ld_row = [&]( int line, int n ) -> C {
             const int g = line / NP, q = line % NP;
             const int nn = n <= N / 2 ? n : N - n;
-            C         av = lds[COLS::at( g * NAH + nn, 2 * q )];
-            C         bv = lds[COLS::at( g * NAH + nn, 2 * q + 1 )];
+            C         av = lds[COLS::at( g * NAH + nn, 2 * q, pc )];
+            C         bv = lds[COLS::at( g * NAH + nn, 2 * q + 1, pc )];
             if ( n > N / 2 )
             {
                 av.y = -av.y;
@@ -569,12 +643,42 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
             return mk<T>( av.x - bv.y, av.y + bv.x );
         };
         auto st_row = [&]( int line, int n, const C &v ) {
+            if constexpr ( staged )
+            {
+                lds[ROWS::at( line, n, PX )] = v;
+                return;
+            }
             const int g = line / NP, q = line % NP;
-            if ( pv[g] )
-                *reinterpret_cast<V2 *>( static_cast<T *>( p.out ) + ro[g] + (int64_t)n * p.ra + 2 * q ) =
-                    V2{ v.x * p.scale, v.y * p.scale };
+            if ( !pv[g] )
+                return;
+            T *dst = static_cast<T *>( p.out ) + ro[g] + (int64_t)n * p.ra + (int64_t)( 2 * q ) * p.rb;
+            if ( p.pair )
+                *reinterpret_cast<V2 *>( dst ) = V2{ v.x * p.scale, v.y * p.scale };
+            else
+            {
+                dst[0]    = v.x * p.scale;
+                dst[p.rb] = v.y * p.scale;
+            }
         };
-        ROWS::template run<0>( lds, twa, ld_row, st_row );
+        ROWS::template run<0>( lds, PX, twa, ld_row, st_row, true, staged, no_hook );
+        if constexpr ( staged )
+        {
+            // the real rows out of the row layout, consecutive threads along n
+            __syncthreads();
+#pragma unroll
+            for ( int i = 0; i < PZT; ++i )
+            {
+                const int e = tid + i * NT, g = e / ( NP * N ), r = e % ( NP * N );
+                if ( ( ITZ % NT == 0 || e < ITZ ) && pv[g] )
+                {
+                    const C v   = lds[ROWS::at( g * NP + r / N, r % N, PX )];
+                    T      *dst = static_cast<T *>( p.out ) + ro[g] + (int64_t)( r % N ) * p.ra +
+                             (int64_t)( 2 * ( r / N ) ) * p.rb;
+                    dst[0]    = v.x * p.scale;
+                    dst[p.rb] = v.y * p.scale;
+                }
+            }
+        }
     }
 }
 

@@ -103,23 +103,28 @@ def test_plane_pass_adjoints(gpu, kind, dims):
     assert abs(a - b) <= 1e-11 * abs(a)
 
 
-# pencil planes (B = x contiguous in both arrays): shape 1 runs the register-resident
-# plane kernel; plane counts that leave the last workgroup partly empty included
-RX_CASES = [("float32", (16, 15, 35, 16)), ("float32", (32, 32, 32, 32)), ("float32", (64, 27, 21, 64)),
-            ("float32", (128, 8, 64, 128)), ("float64", (64, 27, 21, 64)), ("float64", (128, 8, 64, 128))]
+# shape 1 runs the register-resident plane kernel; every layout (pencil: B contiguous,
+# paired real rows; 3D / slab / xzwy: A contiguous), and plane counts that leave the
+# last workgroup partly empty
+RX_CASES = [("pencil", "float32", (16, 15, 35, 16)), ("pencil", "float32", (32, 32, 32, 32)),
+            ("pencil", "float32", (64, 27, 21, 64)), ("pencil", "float32", (128, 8, 64, 128)),
+            ("pencil", "float64", (64, 27, 21, 64)), ("pencil", "float64", (128, 8, 64, 128)),
+            ("3d", "float32", (525, 64, 64)), ("3d", "float64", (512, 32, 32)), ("3d", "float32", (512, 128, 128)),
+            ("slab", "float32", (8, 64, 64, 64)), ("slab", "float64", (4, 128, 128, 128)),
+            ("xzwy", "float32", (64, 32, 16, 64)), ("xzwy", "float64", (32, 32, 32, 32))]
 
 
-@pytest.mark.parametrize("real,dims", RX_CASES)
-def test_plane_register_kernel_bitwise(gpu, real, dims):
+@pytest.mark.parametrize("kind,real,dims", RX_CASES)
+def test_plane_register_kernel_bitwise(gpu, kind, real, dims):
     """Plane-pass variants 1 (register-resident kernel), 8 and 9 (round-robin tile order)
     reproduce variant 0 bitwise: forward, backward and both adjoints (weight modes 1, 2)."""
     import torch
     from fftm_amd import _lib
     L = _lib.lib()
-    f = _make("pencil", dims, real)
+    f = _make(kind, dims, real)
     nf, nb = f.num_passes()
-    print(f"case plane-rx pencil {dims} {real}", flush=True)
-    assert nf == 3 and nb == 3, "plane fusion expected"
+    print(f"case plane-rx {kind} {dims} {real}", flush=True)
+    assert nf == len(dims) - 1 and nb == len(dims) - 1, "plane fusion expected"
     dt = np.float32 if real == "float32" else np.float64
     ct = np.complex64 if real == "float32" else np.complex128
     g = np.random.default_rng(11)
@@ -138,12 +143,16 @@ def test_plane_register_kernel_bitwise(gpu, real, dims):
         f.c2r_adjoint(u, B)
         return X, y, A, B
 
-    ref = run()
-    check_close(from_gpu(ref[0]), _oracle_fwd("pencil", x.astype(np.float64), dims), dt, f"plane-rx fwd {dims}")
-    for var in (1, 8, 9):
+    def set_all(var):
         for bw, n in ((0, nf), (1, nb)):
             for i in range(n):
                 _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
+
+    set_all(0)
+    ref = run()
+    check_close(from_gpu(ref[0]), _oracle_fwd(kind, x.astype(np.float64), dims), dt, f"plane-rx fwd {kind}{dims}")
+    for var in (1, 8, 9):
+        set_all(var)
         out = run()
         for a, b, name in zip(ref, out, ("forward", "backward", "r2c_adjoint", "c2r_adjoint")):
             assert torch.equal(a, b), f"variant {var}: {name} differs from variant 0"
