@@ -315,6 +315,19 @@ __device__ __forceinline__ void dft_runtime( cpx<T> *v, int R, const cpx<T> *__r
 // consecutive tiles (which share cache lines at their edges) are assigned to the
 // workgroups i, i+8, i+16, ... of the same XCD and its L2.
 // ---------------------------------------------------------------------------
+// Chunks of CH consecutive tiles per XCD, the chunks dealt to the XCDs round-robin
+// (workgroup w runs on XCD w % 8).  Workgroups past the last whole round keep their own
+// index, so the mapping stays a permutation of [0, nwg).
+__device__ __forceinline__ int64_t xcd_chunk( int64_t wg, int64_t nwg, int64_t ch )
+{
+    constexpr int64_t NX   = 8;
+    const int64_t     full = nwg / ( NX * ch ) * ( NX * ch );
+    if ( wg >= full )
+        return wg;
+    const int64_t x = wg % NX, k = wg / NX;
+    return ( ( k / ch ) * NX + x ) * ch + k % ch;
+}
+
 __device__ __forceinline__ int64_t xcd_tile( int64_t wg, int64_t nwg )
 {
     constexpr int64_t NX = 8;

@@ -276,13 +276,13 @@ hipError_t launch_plane_one( const pass_rt &r, const tables &tb, const void *src
     p.ci0     = r.ci0;
     p.ci1     = r.ci1;
     p.wmode   = r.wmode;
-    p.order   = ( r.var & VARIANT_RR ) ? 1 : 0;
+    p.order   = variant_order_of( r.var );
     p.scale   = (T)r.scale;
     const bool fwd = r.kind == KIND_R2C;
     // shape 1: the register-resident kernel (square planes); its real rows 2q, 2q+1 move
     // as one pair where B is contiguous, every row start is even and the array aligned
     const void *real = fwd ? src : static_cast<const void *>( dst );
-    const bool  rx   = ( r.var & ~VARIANT_RR ) == 1 && r.N == r.NB;
+    const bool  rx   = variant_shape_of( r.var ) == 1 && r.N == r.NB;
     p.pair           = ( r.rb == 1 && r.ra % 2 == 0 && ( r.pd0 <= 1 || r.ri0 % 2 == 0 ) &&
                ( r.nplanes <= r.pd0 || r.ri1 % 2 == 0 ) &&
                ( reinterpret_cast<uintptr_t>( real ) % ( 2 * sizeof( T ) ) ) == 0 )
@@ -341,8 +341,8 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.vec     = pair_access_bits( r, src, dst );
     p.scale   = (T)r.scale;
     // tile shape (compile-time kernel) and tile order (runtime bit)
-    const int  shape = r.var & ~VARIANT_RR;
-    const int  rr    = ( r.var & VARIANT_RR ) ? VEC_RR : 0;
+    const int  shape = variant_shape_of( r.var );
+    const int  rr    = variant_vec_order( r.var );
     p.vec |= rr;
     if constexpr ( std::is_same<T, float>::value )
     {
@@ -1631,10 +1631,10 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
     {
         // the tile shape follows the tuned forward pass it replaces (shapes 0, 2, 4)
         pre.back().filter = true;
-        const int lv      = lf.var & ~VARIANT_RR;
+        const int lv      = variant_shape_of( lf.var );
         pre.back().var    = h.flt_var >= 0 ? h.flt_var
                                            : ( ( lv == 2 || lv == 4 ) ? lv : ( ( lv == 3 || lv == 5 ) ? 4 : 0 ) ) |
-                                                 ( lf.var & VARIANT_RR );
+                                                 ( lf.var & ~7 );
         post.erase( post.begin() );
     }
     h.flt_pre     = pre;
@@ -1876,7 +1876,7 @@ __global__ void k_count_diff( const uint32_t *a, const uint32_t *b, int64_t n, u
 }
 
 // variants of the fused filter pass: tile shapes 0, 2 and 4, each in both tile orders
-constexpr int FILTER_CANDIDATES[] = { 0, 2, 4, VARIANT_RR, VARIANT_RR | 2, VARIANT_RR | 4 };
+constexpr int FILTER_CANDIDATES[] = { 0, 2, 4, 8, 10, 12, 16, 18, 20 };
 
 bool filter_variant_ok( int var )
 {
@@ -2133,10 +2133,10 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             // (plane passes: shape 0 the LDS kernel, 1 the register kernel; any-radix
             // passes have one shape and their own order)
             const int            nvar    = cur.generic ? 1 : ( cur.plane ? 2 : fixed_variants( cur.N ) );
-            const int            ncand   = cur.generic ? 1 : 2 * nvar;
+            const int            ncand   = cur.generic ? 1 : NUM_ORDERS * nvar;
             for ( int cand = 0; cand < ncand && rc == 0; ++cand )
             {
-                const int var = cand < nvar ? cand : ( cand - nvar ) | VARIANT_RR;
+                const int var = cand % nvar + 8 * ( cand / nvar );
                 if ( cur.iom && !mixed_variant_ok( var ) )
                     continue;
                 one[0].var = var;

@@ -56,13 +56,19 @@ inline bool mixed_supported( int64_t N )
 }
 inline bool mixed_variant_ok( int var )
 {
-    const int v = var & ~VARIANT_RR;
+    const int v = variant_shape_of( var );
     return v == 0 || v == 6 || v == 7;
 }
-// a variant number: a tile shape 0..NUM_VARIANTS-1, plus VARIANT_RR for the tile order
+// a variant number: a tile shape 0..NUM_VARIANTS-1 plus 8 * the tile order
 inline bool variant_number_ok( int var )
 {
-    return var >= 0 && ( var & ~VARIANT_RR ) < NUM_VARIANTS;
+    return var >= 0 && variant_shape_of( var ) < NUM_VARIANTS && variant_order_of( var ) < NUM_ORDERS;
+}
+// the order bits of pass_params::vec for a variant number
+inline int variant_vec_order( int var )
+{
+    const int o = variant_order_of( var );
+    return o == 1 ? VEC_RR : ( o == 2 ? VEC_CHUNK : 0 );
 }
 hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, int iom,
                                   hipStream_t s );

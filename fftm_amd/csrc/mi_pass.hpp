@@ -300,12 +300,27 @@ constexpr int geom_pa( int N, int pad ) { return pad ? N + ( N >> pad ) : N; }
 //   7: the tile of 0 exchanged in two rounds, registers capped for 4 waves per SIMD
 //      (some spilling; wins on the longest contiguous R2C lines)
 constexpr int NUM_VARIANTS = 8;
-// Bit 3 of a pass's variant number selects the tile order, a runtime choice: set, tile t
-// runs as workgroup t, so consecutive tiles go to different XCDs; clear, each XCD takes a
-// contiguous range of tiles (xcd_tile), so neighbouring tiles share one L2.  The host
-// passes the bit to the kernel as bit 2 of pass_params::vec.
-constexpr int VARIANT_RR = 8;
-constexpr int VEC_RR     = 4;
+// A pass's variant number is shape + 8 * order.  The tile order is a runtime choice:
+//   0: each XCD takes one contiguous range of tiles (xcd_tile), so neighbouring tiles
+//      share one L2;
+//   1: tile t runs as workgroup t, so consecutive tiles go to different XCDs;
+//   2: chunks of ORDER_CHUNK consecutive tiles per XCD, dealt round-robin (xcd_chunk).
+// The host passes the order to the kernel in pass_params::vec (VEC_RR, VEC_CHUNK).
+constexpr int NUM_ORDERS  = 3;
+constexpr int ORDER_CHUNK = 8;
+constexpr int VEC_RR      = 4;
+constexpr int VEC_CHUNK   = 8;
+constexpr int variant_shape_of( int var ) { return var & 7; }
+constexpr int variant_order_of( int var ) { return var >> 3; }
+
+__device__ __forceinline__ int64_t ordered_tile( int order_bits )
+{
+    if ( order_bits & VEC_RR )
+        return (int64_t)blockIdx.x;
+    if ( order_bits & VEC_CHUNK )
+        return xcd_chunk( blockIdx.x, gridDim.x, ORDER_CHUNK );
+    return xcd_tile( blockIdx.x, gridDim.x );
+}
 
 constexpr bool variant_split( int var ) { return var >= 6 || var == 3; }
 constexpr int  variant_shape( int var ) { return var >= 6 ? 0 : ( var == 3 ? 4 : var ); }
@@ -1078,7 +1093,7 @@ struct pass_kernel
         const int tid = threadIdx.x;
         x.c          = LOADB ? tid % TW : tid / TPC;
         x.t          = LOADB ? tid / TW : tid % TPC;
-        x.tile       = ( p.vec & VEC_RR ) ? (int64_t)blockIdx.x : xcd_tile( blockIdx.x, gridDim.x );
+        x.tile       = ordered_tile( p.vec );
         const int64_t col = ( x.tile * TW + x.c ) * CPT;
         x.va         = col < p.ncols;
         x.vb         = CPT == 2 && col + 1 < p.ncols;
