@@ -276,7 +276,7 @@ hipError_t launch_plane_one( const pass_rt &r, const tables &tb, const void *src
     p.ci0     = r.ci0;
     p.ci1     = r.ci1;
     p.wmode   = r.wmode;
-    p.order   = variant_order_of( r.var );
+    p.order   = variant_vec_order( r.var );
     p.scale   = (T)r.scale;
     const bool fwd = r.kind == KIND_R2C;
     // shape 1: the register-resident kernel (square planes); its real rows 2q, 2q+1 move
@@ -1876,7 +1876,7 @@ __global__ void k_count_diff( const uint32_t *a, const uint32_t *b, int64_t n, u
 }
 
 // variants of the fused filter pass: tile shapes 0, 2 and 4, each in both tile orders
-constexpr int FILTER_CANDIDATES[] = { 0, 2, 4, 8, 10, 12, 16, 18, 20 };
+constexpr int FILTER_CANDIDATES[] = { 0, 2, 4, 8, 10, 12, 16, 18, 20, 24, 26, 28 };
 
 bool filter_variant_ok( int var )
 {

@@ -67,8 +67,8 @@ inline bool variant_number_ok( int var )
 // the order bits of pass_params::vec for a variant number
 inline int variant_vec_order( int var )
 {
-    const int o = variant_order_of( var );
-    return o == 1 ? VEC_RR : ( o == 2 ? VEC_CHUNK : 0 );
+    static constexpr int bits[NUM_ORDERS] = { 0, VEC_RR, VEC_CHUNK8, VEC_CHUNK32 };
+    return bits[variant_order_of( var )];
 }
 hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, int iom,
                                   hipStream_t s );

@@ -304,12 +304,14 @@ constexpr int NUM_VARIANTS = 8;
 //   0: each XCD takes one contiguous range of tiles (xcd_tile), so neighbouring tiles
 //      share one L2;
 //   1: tile t runs as workgroup t, so consecutive tiles go to different XCDs;
-//   2: chunks of ORDER_CHUNK consecutive tiles per XCD, dealt round-robin (xcd_chunk).
-// The host passes the order to the kernel in pass_params::vec (VEC_RR, VEC_CHUNK).
-constexpr int NUM_ORDERS  = 3;
-constexpr int ORDER_CHUNK = 8;
+//   2: chunks of 8 consecutive tiles per XCD, dealt round-robin (xcd_chunk);
+//   3: chunks of 32.
+// The host passes the order to the kernel in pass_params::vec (VEC_RR, VEC_CHUNK8,
+// VEC_CHUNK32).
+constexpr int NUM_ORDERS  = 4;
 constexpr int VEC_RR      = 4;
-constexpr int VEC_CHUNK   = 8;
+constexpr int VEC_CHUNK8  = 8;
+constexpr int VEC_CHUNK32 = 16;
 constexpr int variant_shape_of( int var ) { return var & 7; }
 constexpr int variant_order_of( int var ) { return var >> 3; }
 
@@ -317,8 +319,10 @@ __device__ __forceinline__ int64_t ordered_tile( int order_bits )
 {
     if ( order_bits & VEC_RR )
         return (int64_t)blockIdx.x;
-    if ( order_bits & VEC_CHUNK )
-        return xcd_chunk( blockIdx.x, gridDim.x, ORDER_CHUNK );
+    if ( order_bits & VEC_CHUNK8 )
+        return xcd_chunk( blockIdx.x, gridDim.x, 8 );
+    if ( order_bits & VEC_CHUNK32 )
+        return xcd_chunk( blockIdx.x, gridDim.x, 32 );
     return xcd_tile( blockIdx.x, gridDim.x );
 }
 

@@ -32,7 +32,7 @@ struct plane_params
     int64_t       nplanes, d0;      // planes; batch dim 0 extent (dim 1 = rest)
     int64_t       ri0, ri1, ci0, ci1; // real / complex strides of the two batch dims
     int32_t       wmode;            // real-axis weights, as pass_params::wmode
-    int32_t       order;            // tile order, as the pass kernel's (0 per-XCD, 1 round-robin, 2 chunks)
+    int32_t       order;            // pass_params::vec order bits (VEC_RR, VEC_CHUNK8, VEC_CHUNK32)
     int32_t       pair;             // k_plane_rx: real rows 2q, 2q+1 move as one aligned pair
     int32_t       pad_;
     T             scale;
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__( ( plane_geom<NA, NB, (int)sizeof( cpx<T> ), P
     C            *twa   = lds + G::TWOFF;
     C            *twb   = twa + NA;
     const int     tid   = threadIdx.x;
-    const int64_t first = ordered_tile( p.order == 1 ? VEC_RR : ( p.order == 2 ? VEC_CHUNK : 0 ) ) * PPW;
+    const int64_t first = ordered_tile( p.order ) * PPW;
     int64_t       ro[PPW], co[PPW];
     bool          pv[PPW];
 #pragma unroll
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
     C            *twa   = lds + G::ELEMS;
     C            *twb   = twa + N;
     const int     tid   = threadIdx.x;
-    const int64_t first = ordered_tile( p.order == 1 ? VEC_RR : ( p.order == 2 ? VEC_CHUNK : 0 ) ) * PPW;
+    const int64_t first = ordered_tile( p.order ) * PPW;
     int64_t       ro[PPW], co[PPW];
     bool          pv[PPW];
 #pragma unroll

@@ -173,9 +173,10 @@ int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *b
  * passed; the pass reads and writes exactly what it does inside that call.  Used to
  * time individual kernels; it has no counterpart in the reference. */
 int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
-/* Variant number (0..23) of one pass's kernel: a tile shape 0..7 plus 8 * the tile
+/* Variant number (0..31) of one pass's kernel: a tile shape 0..7 plus 8 * the tile
  * order.  Orders: 0 one contiguous range of tiles per XCD, 1 the tiles dealt to the
- * XCDs round-robin, 2 chunks of 8 consecutive tiles dealt round-robin.  Shapes:
+ * XCDs round-robin, 2 and 3 chunks of 8 and of 32 consecutive tiles dealt
+ * round-robin.  Shapes:
  * 0 default, 1 half the columns per workgroup, 2 twice the threads per column, 3 the
  * tile of 4 exchanged through LDS one component at a time, 4 twice the columns
  * (256-byte row segments, up to 160 KB of LDS), 5 twice the columns and the threads

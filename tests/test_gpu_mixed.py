@@ -115,7 +115,7 @@ def test_h16_variants_bitwise_and_autotune(gpu):
     f.forward(u, X0)
     f.backward(X0, y0)
     nf, nb = f.num_passes()
-    for var in (0, 6, 7, 8, 14, 15, 16, 22, 23):
+    for var in (0, 6, 7, 8, 14, 15, 16, 22, 23, 24, 30, 31):
         for bw, n in ((0, nf), (1, nb)):
             for i in range(n):
                 _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
@@ -126,7 +126,7 @@ def test_h16_variants_bitwise_and_autotune(gpu):
     assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 1, 3) == 16
     assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 1, 11) == 16
     chosen = f.autotune(warmup=1, iterations=3)
-    assert set(chosen.values()) <= {0, 6, 7, 8, 14, 15, 16, 22, 23}
+    assert set(chosen.values()) <= {0, 6, 7, 8, 14, 15, 16, 22, 23, 24, 30, 31}
     X, y = f.complex_array(), f.real_array()
     f.forward(u, X)
     f.backward(X, y)

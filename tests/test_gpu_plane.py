@@ -117,8 +117,8 @@ RX_CASES = [("pencil", "float32", (16, 15, 35, 16)), ("pencil", "float32", (32, 
 @pytest.mark.parametrize("kind,real,dims", RX_CASES)
 def test_plane_register_kernel_bitwise(gpu, kind, real, dims):
     """Plane-pass variants 1 (register-resident kernel), 8, 9 (round-robin tile order),
-    16 and 17 (chunked tile order) reproduce variant 0 bitwise: forward, backward and
-    both adjoints (weight modes 1, 2)."""
+    16, 17, 24 and 25 (chunked tile orders) reproduce variant 0 bitwise: forward,
+    backward and both adjoints (weight modes 1, 2)."""
     import torch
     from fftm_amd import _lib
     L = _lib.lib()
@@ -152,7 +152,7 @@ def test_plane_register_kernel_bitwise(gpu, kind, real, dims):
     set_all(0)
     ref = run()
     check_close(from_gpu(ref[0]), _oracle_fwd(kind, x.astype(np.float64), dims), dt, f"plane-rx fwd {kind}{dims}")
-    for var in (1, 8, 9, 16, 17):
+    for var in (1, 8, 9, 16, 17, 24, 25):
         set_all(var)
         out = run()
         for a, b, name in zip(ref, out, ("forward", "backward", "r2c_adjoint", "c2r_adjoint")):

@@ -43,14 +43,14 @@ def test_autotune_cache_round_trip(gpu, tmp_path, dt, dims):
     f.backward(X0, y0)
     g0 = f.real_array()
     f.filter(u, g0, "poisson", scale=-1.0 / u.numel())
-    # variant numbers: shapes 0..7 plus 8 * the tile order (0..2); anything else is rejected
-    assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 0, 24) != 0
+    # variant numbers: shapes 0..7 plus 8 * the tile order (0..3); anything else is rejected
+    assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 0, 32) != 0
     assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 0, -1) != 0
     cache = str(tmp_path / "tile.cache")
     chosen = f.autotune(warmup=1, iterations=3, cache_path=cache)
     nf, nb = f.num_passes()
     assert len(chosen) == nf + nb
-    assert all(0 <= v < 24 for v in chosen.values())
+    assert all(0 <= v < 32 for v in chosen.values())
     X, y = f.complex_array(), f.real_array()
     f.forward(u, X)
     f.backward(X, y)

@@ -1,4 +1,4 @@
-"""Every variant of every pass (tile shape 0..7, each in the three tile orders) produces
+"""Every variant of every pass (tile shape 0..7, each in the four tile orders) produces
 bitwise the same result as the default (the per-column arithmetic depends on neither the
 tile shape nor the order the tiles run in), so the tuner can pick any of them without a
 parity risk."""
@@ -31,7 +31,7 @@ def test_variants_bitwise_equal(gpu, dt, kind, dims):
     f.forward(u, X0)
     f.backward(X0, y0)
     nf, nb = f.num_passes()
-    for var in range(1, 24):
+    for var in range(1, 32):
         for bw, n in ((0, nf), (1, nb)):
             for i in range(n):
                 _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
