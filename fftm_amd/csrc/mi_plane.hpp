@@ -511,7 +511,7 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
     const int      pc      = lf_cols ? G::PCO : G::PC;
     constexpr int ITZ  = PPW * NP * N; // staged real-row elements (pairs of reals)
     constexpr int PZT  = ( ITZ + NT - 1 ) / NT;
-    __shared__ C  lds[G::TOTAL];
+    __shared__ __align__( 16 ) C lds[G::TOTAL]; // 16-byte pair reads of S below
     C            *twa   = lds + G::ELEMS;
     C            *twb   = twa + N;
     const int     tid   = threadIdx.x;
@@ -616,15 +616,31 @@ __global__ void __launch_bounds__( ( plane_rx_geom<N, (int)sizeof( cpx<T> )>::NT
                 return mk<T>( 0, 0 );
             return static_cast<const C *>( p.in )[co[g] + (int64_t)k * p.ca + (int64_t)b * p.cb];
         };
-        auto st_col = [&]( int col, int b, const C &v ) { lds[COLS::at( col, b, pc )] = v; };
+        // S in natural order.  Where threads walk the butterflies of a column (pc even) and
+        // the rows take consecutive q, S is stored unpadded (pitch pc): the pair S[2q], S[2q+1]
+        // of a column is then one aligned 16-byte LDS read (FP32), and 16 threads reading
+        // consecutive pairs cover all banks once
+        const bool pair_s = !lf_cols && !staged && sizeof( T ) == 4;
+        auto       s_at   = [&]( int col, int b ) { return pair_s ? col * pc + b : COLS::at( col, b, pc ); };
+        auto       st_col = [&]( int col, int b, const C &v ) { lds[s_at( col, b )] = v; };
         COLS::template run<0>( lds, pc, twb, ld_col, st_col, lf_cols, true, copy_twiddles );
         __syncthreads();
         // rows, inverse: Z[q][n] = A~[n] + i B~[n] (Hermitian extension) on the way in
         auto ld_row = [&]( int line, int n ) -> C {
             const int g = line / NP, q = line % NP;
             const int nn = n <= N / 2 ? n : N - n;
-            C         av = lds[COLS::at( g * NAH + nn, 2 * q, pc )];
-            C         bv = lds[COLS::at( g * NAH + nn, 2 * q + 1, pc )];
+            C         av, bv;
+            if ( pair_s )
+            {
+                const f32x4 w = *reinterpret_cast<const f32x4 *>( &lds[s_at( g * NAH + nn, 2 * q )] );
+                av            = mk<T>( (T)w.x, (T)w.y );
+                bv            = mk<T>( (T)w.z, (T)w.w );
+            }
+            else
+            {
+                av = lds[s_at( g * NAH + nn, 2 * q )];
+                bv = lds[s_at( g * NAH + nn, 2 * q + 1 )];
+            }
             if ( n > N / 2 )
             {
                 av.y = -av.y;
