@@ -64,6 +64,17 @@ def _traffic_summary(kernel_key):
 TUNING_CACHE = None  # --tuning-cache PATH
 
 
+def timing_stats(samples):
+    """Per-pair statistics in ms: mean and population standard deviation as the reference's
+    compute_timing_statistics (fft_benchmark_common.h:41-61), plus median, min and p95."""
+    a = np.asarray(samples, dtype=np.float64)
+    if a.size == 0:
+        return {}
+    return {"mean": round(float(a.mean()), 4), "stddev": round(float(a.std()), 4),
+            "median": round(float(np.median(a)), 4), "min": round(float(a.min()), 4),
+            "p95": round(float(np.percentile(a, 95)), 4), "samples": int(a.size)}
+
+
 def tune_handle(f, warmup, iterations):
     """Tune f's passes, or load them from TUNING_CACHE when it already holds this problem
     (the reference's KEY=VALUE cache: runs that must see the same variants share it)."""
@@ -168,16 +179,22 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
     dom_ms = float(np.mean([ev0[s].elapsed_time(ev1[s]) for s in range(steps)]))
     # per-pass breakdown (untimed, events between all passes)
     per_pass_ms = per_pass(steps)
-    # also the plain call (forward + backward API), no events in between
+    # also the plain calls (forward + backward API), with a HIP event between consecutive
+    # pairs only: the reference's protocol (events around forward + backward,
+    # test_benchmark_ffts_3D.cu:65-98) for the per-pair statistics
+    pev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    for _ in range(steps):
+    pev[0].record(stream)
+    for k in range(steps):
         pair()
+        pev[k + 1].record(stream)
     torch.cuda.synchronize()
     api_pair_s = (time.perf_counter() - t2) / steps
+    pair_stats = timing_stats([pev[k].elapsed_time(pev[k + 1]) for k in range(steps)])
 
     res = {"pair_s": pair_s, "api_pair_s": api_pair_s, "per_pass_ms": per_pass_ms, "pass_bytes": pbytes,
-           "dom": dom, "dom_ms": dom_ms,
+           "dom": dom, "dom_ms": dom_ms, "pair_stats_ms": pair_stats,
            "passes": passes, "variants": {f"{'bwd' if b else 'fwd'}{i}": v for (b, i), v in variants.items()}}
     if check == "gpu64":
         pair()
@@ -536,6 +553,7 @@ def main():
         "hbm_gbps_alg": round(balg / pair_s / 1e9, 1),
         "b_alg_bytes": balg,
         "api_ms_per_pair": round(r3["api_pair_s"] * 1e3, 4),
+        "api_pair_timing_ms": r3["pair_stats_ms"],
         "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r3["per_pass_ms"].items()},
         "pass_variants": r3["variants"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -556,6 +574,7 @@ def main():
             "gflops": round(_flops_pair(N4) / r4["api_pair_s"] / 1e9, 2),
             "ms_per_pair": round(r4["api_pair_s"] * 1e3, 4),
             "ms_per_pair_timed_region": round(r4["pair_s"] * 1e3, 4),
+            "pair_timing_ms": r4["pair_stats_ms"],
             "hbm_gbps_alg": round(_b_alg(d4, 4) / r4["api_pair_s"] / 1e9, 1),
             "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r4["per_pass_ms"].items()},
             "pass_variants": r4["variants"],

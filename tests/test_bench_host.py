@@ -1,0 +1,41 @@
+"""Host-side pieces of bench.py that need no GPU: the per-pair timing statistics follow
+the reference's compute_timing_statistics (source/tests/detail/fft_benchmark_common.h:
+41-61: arithmetic mean, population standard deviation), and `--gpus` above 1 exits with
+a message."""
+
+import importlib.util
+import os
+import statistics
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_timing_stats_match_reference_definitions():
+    samples = [1.3, 1.25, 1.41, 1.33, 1.29, 1.36, 1.31, 1.5, 1.28, 1.3]
+    st = _bench().timing_stats(samples)
+    assert abs(st["mean"] - statistics.fmean(samples)) < 1e-4
+    # population (divide by n), as the reference; the sample stddev (n - 1) would differ
+    assert abs(st["stddev"] - statistics.pstdev(samples)) < 1e-4
+    assert abs(st["stddev"] - statistics.stdev(samples)) > 1e-3
+    assert st["median"] == round(statistics.median(samples), 4)
+    assert st["min"] == min(samples) and st["samples"] == len(samples)
+    srt = sorted(samples)
+    # numpy's default (linear) 95th percentile lies between the two largest samples here
+    assert srt[-2] <= st["p95"] <= srt[-1]
+    assert _bench().timing_stats([]) == {}
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
