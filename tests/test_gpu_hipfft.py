@@ -1,7 +1,7 @@
 """Our transforms against the reference's own backend on the same inputs: the system
 hipFFT (rocFFT), reached through torch.fft on ROCm.  This is the comparison the
 reference's ffts makes when built for HIP (hipfft_wrap.h:303-417); the FP64 oracle
-stays the accuracy judge, this pins "the same answer as the reference's library"."""
+stays the accuracy reference, this pins "the same answer as the reference's library"."""
 
 import numpy as np
 import pytest

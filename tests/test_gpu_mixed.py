@@ -4,7 +4,7 @@ The half spectrum between passes is stored as FP16 pairs with one power-of-two
 exponent per stored line, chosen so that the line's largest component lands in
 [2^14, 2^15).  Each storage rounds with unit roundoff 2^-11, so the forward spectrum
 (two roundings) stays within a few 1e-4 relative L2 of the FP64 oracle; the plain
-FP32 path is the accuracy judge.  Scaling the input by a power of two scales every
+FP32 path is the accuracy reference.  Scaling the input by a power of two scales every
 stored exponent by the same amount, so results scale bitwise exactly -- that pins the
 exponent bookkeeping independently of the FP16 rounding."""
 
