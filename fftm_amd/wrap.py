@@ -235,12 +235,14 @@ class fft:
 
 class fft_wrap_many:
     """Named-plan container (fft_wrap_many.h:25-1927): the plans ffts uses, raw-pointer
-    execution, plan sequences executed at element offsets (fft_wrap_many.h:435-780) and
-    plans bound to their own non-blocking streams (fft_wrap_many.h:396-433).
+    execution, plan sequences executed at element offsets (fft_wrap_many.h:435-780),
+    plans bound to their own non-blocking streams (fft_wrap_many.h:396-433), C2C plan
+    arrays (one plan per element offset, each on its own stream; :784-868, 1260-1530)
+    and R2C/C2R plan bundles (lanes of 2D forward/inverse plan pairs with their own
+    streams and work areas; :188-237, 310-378, 1815-1855).
 
-    The C2C plan arrays and R2C/C2R bundles of the MPI frontend (fft_wrap_many.h:188-237,
-    1260-1605) raise ``NotImplementedError``; C++ callers get them from the reference's
-    own container on top of ``fftm::wrap::mi::fft``."""
+    The handle and stream "tokens" and the minimal-reference plan arrays, which expose
+    hipFFT handles for diagnostics (:1270-1365), raise ``NotImplementedError``."""
 
     def __init__(self, real):
         self.real = real
@@ -252,6 +254,8 @@ class fft_wrap_many:
         self._work = None
         self._memory_profiler = None
         self._memory_profile_prefix = ""
+        self._plan_arrays = []
+        self._bundles = []
 
     # -- plan creation (fft_wrap_many.h:120-186) ----------------------------
     def add_plan(self, D, name: str, n, inembed, istride, idist, onembed, ostride, odist, batch):
@@ -293,12 +297,15 @@ class fft_wrap_many:
             self._work = torch.empty(self._work_area_size, dtype=torch.uint8, device="cuda")
             for p in self._plans.values():
                 p.set_work_area(self._work)
+            self._bind_bundle_work_areas(device_ptr(self._work))
         self._update_memory_profile()
         self._activated = True
 
     def activate_work_size(self) -> int:
         for p in self._plans.values():
             self._work_area_size = max(self._work_area_size, p.get_work_size())
+        for b in self._bundles:
+            self._work_area_size = max(self._work_area_size, self._bundle_work_size(b))
         return self._work_area_size
 
     def set_external_work_area(self, work_area):
@@ -307,6 +314,7 @@ class fft_wrap_many:
                              "local work_area.")
         for p in self._plans.values():
             p.set_work_area(work_area)
+        self._bind_bundle_work_areas(device_ptr(work_area))
         self._external_activated = True
         self._update_memory_profile()
 
@@ -362,6 +370,14 @@ class fft_wrap_many:
         for p in self._plans.values():
             p.destroy()
         self._plans.clear()
+        for arr in getattr(self, "_plan_arrays", []):
+            for p in arr["plans"]:
+                p.destroy()
+        for b in getattr(self, "_bundles", []):
+            for p in b["forward"] + b["inverse"]:
+                p.destroy()
+        self._plan_arrays = []
+        self._bundles = []
         self._work = None
         self._work_area_size = 0
         self._activated = False
@@ -453,13 +469,209 @@ class fft_wrap_many:
     exec_plan_sequence_offsets_tight_direction = exec_plan_sequence_offsets_direction
     exec_plan_sequence_offsets_opaque_direction = exec_plan_sequence_offsets_direction
 
-    # -- fftm-only surface ---------------------------------------------------
-    def _fftm_only(self, *a, **k):
-        raise NotImplementedError("fft_wrap_many: C2C plan arrays and R2C/C2R bundles belong to the MPI fftm "
-                                  "frontend; use the reference's fft_wrap_many over fftm::wrap::mi::fft in C++")
+    # -- C2C plan arrays (fft_wrap_many.h:784-868, 1260-1530) -------------------
+    # One C2C plan per element offset, each on its own non-blocking stream (or on the
+    # caller's streams); the work areas are bound explicitly, plan i at base + i * stride.
+    @staticmethod
+    def invalid_c2c_plan_array_id() -> int:
+        return (1 << 64) - 1
 
-    make_r2c_c2r_plan_bundle_2D = _fftm_only
-    make_c2c_plan_array = _fftm_only
+    def _new_c2c_plan_array(self, what, n, inembed, istride, idist, onembed, ostride, odist, batch, offsets,
+                            streams=None):
+        if self._activated:
+            raise LogicError(f"fft_wrap_many::{what}: cannot add plan arrays after activation.")
+        arr = {"plans": [], "streams": [], "external_streams": [], "offsets": [int(o) for o in offsets],
+               "work_stride": 0, "descriptor": None}
+        for i in range(len(arr["offsets"])):
+            plan = fft(self.real, direction.C2CF, [n], [inembed], istride, idist, [onembed], ostride, odist, batch)
+            if streams is None:
+                st = runtime_api.create_nonblocking_stream()
+                arr["streams"].append(st)
+            else:
+                st = streams[i]
+                arr["external_streams"].append(st)
+            plan.set_stream(st)
+            arr["work_stride"] = max(arr["work_stride"], plan.get_work_size())
+            if i == 0:
+                arr["descriptor"] = plan.descriptor()
+            arr["plans"].append(plan)
+        self._plan_arrays.append(arr)
+        return len(self._plan_arrays) - 1
+
+    def make_c2c_plan_array_1D(self, n, inembed, istride, idist, onembed, ostride, odist, batch, offsets) -> int:
+        return self._new_c2c_plan_array("make_c2c_plan_array_1D", n, inembed, istride, idist, onembed, ostride,
+                                        odist, batch, offsets)
+
+    def make_raw_c2c_plan_array_1D(self, n, inembed, istride, idist, onembed, ostride, odist, batch,
+                                   offsets) -> int:
+        # the reference's raw-handle arrays dispatch the same plans through hipFFT handles
+        return self._new_c2c_plan_array("make_raw_c2c_plan_array_1D", n, inembed, istride, idist, onembed,
+                                        ostride, odist, batch, offsets)
+
+    def make_raw_c2c_plan_array_1D_with_streams(self, n, inembed, istride, idist, onembed, ostride, odist, batch,
+                                                offsets, streams) -> int:
+        if len(streams) < len(offsets):
+            raise LogicError("fft_wrap_many::make_raw_c2c_plan_array_1D_with_streams: stream count mismatch")
+        return self._new_c2c_plan_array("make_raw_c2c_plan_array_1D_with_streams", n, inembed, istride, idist,
+                                        onembed, ostride, odist, batch, offsets, streams)
+
+    def _plan_array(self, array_id):
+        if array_id < 0 or array_id >= len(self._plan_arrays):
+            raise IndexError("fft_wrap_many: invalid C2C plan-array id")
+        return self._plan_arrays[array_id]
+
+    def c2c_plan_array_size(self, array_id) -> int:
+        return len(self._plan_array(array_id)["plans"])
+
+    def c2c_plan_array_work_size(self, array_id) -> int:
+        return self._plan_array(array_id)["work_stride"]
+
+    def c2c_plan_array_descriptor(self, array_id) -> fft_plan_descriptor:
+        return self._plan_array(array_id)["descriptor"]
+
+    def bind_c2c_plan_array_work_areas(self, array_id, base_work_area, work_stride_bytes: int):
+        arr = self._plan_array(array_id)
+        stride = max(int(work_stride_bytes), arr["work_stride"])
+        base = device_ptr(base_work_area)
+        for i, plan in enumerate(arr["plans"]):
+            plan.set_work_area(base + i * stride if base else 0)
+        arr["bound_stride"] = stride
+        arr["work_ref"] = base_work_area
+
+    def bind_c2c_plan_array_streams(self, array_id, streams):
+        arr = self._plan_array(array_id)
+        if len(streams) < len(arr["plans"]):
+            raise LogicError("fft_wrap_many::bind_c2c_plan_array_streams: stream count mismatch")
+        arr["external_streams"] = list(streams[:len(arr["plans"])])
+        for plan, st in zip(arr["plans"], arr["external_streams"]):
+            plan.set_stream(st)
+
+    def synchronize_c2c_plan_array_streams(self, array_id):
+        arr = self._plan_array(array_id)
+        for st in (arr["external_streams"] or arr["streams"]):
+            runtime_api.stream_synchronize(st)
+
+    def _run_plan_array(self, array_id, exec_dir, in_offsets, out_offsets, inp, out, sync):
+        arr = self._plan_array(array_id)
+        ib, ob = inp.element_size(), out.element_size()
+        ip, op = device_ptr(inp), device_ptr(out)
+        d = direction(exec_dir)
+        for plan, io, oo in zip(arr["plans"], in_offsets, out_offsets):
+            plan._exec(d, ip + int(io) * ib, op + int(oo) * ob, sync=sync)
+
+    def exec_c2c_plan_array_direction(self, array_id, exec_dir, inp, out):
+        offs = self._plan_array(array_id)["offsets"]
+        self._run_plan_array(array_id, exec_dir, offs, offs, inp, out, True)
+
+    def exec_c2c_plan_array_direction_no_sync(self, array_id, exec_dir, inp, out):
+        offs = self._plan_array(array_id)["offsets"]
+        self._run_plan_array(array_id, exec_dir, offs, offs, inp, out, False)
+
+    def exec_c2c_plan_array_direction_dual_offsets_no_sync(self, array_id, exec_dir, in_offsets, out_offsets,
+                                                           inp, out):
+        n = len(self._plan_array(array_id)["plans"])
+        if n != len(in_offsets) or n != len(out_offsets):
+            raise LogicError("fft_wrap_many::exec_c2c_plan_array_direction_dual_offsets_no_sync: offset count "
+                             "mismatch")
+        self._run_plan_array(array_id, exec_dir, in_offsets, out_offsets, inp, out, False)
+
+    # -- R2C/C2R plan bundles (fft_wrap_many.h:188-237, 310-378, 1815-1855) -----
+    @staticmethod
+    def invalid_r2c_c2r_plan_bundle_id() -> int:
+        return (1 << 64) - 1
+
+    def make_r2c_c2r_plan_bundle_2D(self, n0, n1, forward_inembed0, forward_inembed1, forward_istride,
+                                    forward_idist, forward_onembed0, forward_onembed1, forward_ostride,
+                                    forward_odist, forward_batch, inverse_inembed0, inverse_inembed1,
+                                    inverse_istride, inverse_idist, inverse_onembed0, inverse_onembed1,
+                                    inverse_ostride, inverse_odist, inverse_batch, lane_count) -> int:
+        if self._activated or self._external_activated:
+            raise LogicError("fft_wrap_many::make_r2c_c2r_plan_bundle_2D: cannot add plan bundles after activation.")
+        if int(lane_count) <= 0:
+            raise ValueError("fft_wrap_many::make_r2c_c2r_plan_bundle_2D: lane count must be positive")
+        b = {"forward": [], "inverse": [], "streams": [], "work_stride": 0, "bound_stride": 0}
+        for _ in range(int(lane_count)):
+            st = runtime_api.create_nonblocking_stream()
+            f = fft(self.real, direction.R2C, [n0, n1], [forward_inembed0, forward_inembed1], forward_istride,
+                    forward_idist, [forward_onembed0, forward_onembed1], forward_ostride, forward_odist,
+                    forward_batch)
+            g = fft(self.real, direction.C2R, [n0, n1], [inverse_inembed0, inverse_inembed1], inverse_istride,
+                    inverse_idist, [inverse_onembed0, inverse_onembed1], inverse_ostride, inverse_odist,
+                    inverse_batch)
+            f.set_stream(st)
+            g.set_stream(st)
+            b["work_stride"] = max(b["work_stride"], f.get_work_size(), g.get_work_size())
+            b["forward"].append(f)
+            b["inverse"].append(g)
+            b["streams"].append(st)
+        self._bundles.append(b)
+        return len(self._bundles) - 1
+
+    @staticmethod
+    def _aligned_bundle_stride(nbytes: int) -> int:
+        return 0 if nbytes == 0 else (nbytes + 255) // 256 * 256
+
+    def _bundle_work_size(self, b) -> int:
+        return self._aligned_bundle_stride(b["work_stride"]) * len(b["streams"])
+
+    def _bind_bundle_work_areas(self, base: int):
+        for b in self._bundles:
+            stride = self._aligned_bundle_stride(b["work_stride"])
+            for lane in range(len(b["streams"])):
+                w = base + lane * stride if base else 0
+                b["forward"][lane].set_work_area(w)
+                b["inverse"][lane].set_work_area(w)
+            b["bound_stride"] = stride
+
+    def _bundle(self, bundle_id):
+        if bundle_id < 0 or bundle_id >= len(self._bundles):
+            raise IndexError("fft_wrap_many: invalid R2C/C2R plan-bundle id")
+        return self._bundles[bundle_id]
+
+    def r2c_c2r_plan_bundle_lane_count(self, bundle_id) -> int:
+        return len(self._bundle(bundle_id)["streams"])
+
+    def r2c_c2r_plan_bundle_work_size(self, bundle_id) -> int:
+        return self._bundle_work_size(self._bundle(bundle_id))
+
+    def _bundle_lane(self, bundle_id, lane, key, what):
+        b = self._bundle(bundle_id)
+        if lane < 0 or lane >= len(b[key]):
+            raise IndexError(f"fft_wrap_many: R2C/C2R plan-bundle {what} lane is out of range")
+        return b, b[key][lane]
+
+    def exec_r2c_c2r_plan_bundle_forward_lane_no_sync(self, bundle_id, lane, in_offset, out_offset, inp, out):
+        _, plan = self._bundle_lane(bundle_id, lane, "forward", "forward")
+        plan._exec(direction.R2C, device_ptr(inp) + int(in_offset) * inp.element_size(),
+                   device_ptr(out) + int(out_offset) * out.element_size(), sync=False)
+
+    def exec_r2c_c2r_plan_bundle_inverse_lane_no_sync(self, bundle_id, lane, in_offset, out_offset, inp, out):
+        _, plan = self._bundle_lane(bundle_id, lane, "inverse", "inverse")
+        plan._exec(direction.C2R, device_ptr(inp) + int(in_offset) * inp.element_size(),
+                   device_ptr(out) + int(out_offset) * out.element_size(), sync=False)
+
+    def r2c_c2r_plan_bundle_lane_ready(self, bundle_id, lane) -> bool:
+        _, st = self._bundle_lane(bundle_id, lane, "streams", "readiness")
+        return runtime_api.stream_ready(st)
+
+    def synchronize_r2c_c2r_plan_bundle_lane(self, bundle_id, lane):
+        _, st = self._bundle_lane(bundle_id, lane, "streams", "synchronization")
+        runtime_api.stream_synchronize(st)
+
+    def synchronize_r2c_c2r_plan_bundle(self, bundle_id):
+        for st in self._bundle(bundle_id)["streams"]:
+            runtime_api.stream_synchronize(st)
+
+    # -- hipFFT handle diagnostics -------------------------------------------
+    def _handle_diagnostics(self, *a, **k):
+        raise NotImplementedError("fft_wrap_many: handle/stream tokens and minimal-reference plan arrays expose "
+                                  "hipFFT handles; this backend has none")
+
+    c2c_plan_array_first_handle_token = _handle_diagnostics
+    c2c_plan_array_last_handle_token = _handle_diagnostics
+    c2c_plan_array_first_stream_token = _handle_diagnostics
+    c2c_plan_array_last_stream_token = _handle_diagnostics
+    make_minimal_reference_c2c_plan_array_1D = _handle_diagnostics
 
     def __del__(self):
         try:
