@@ -662,16 +662,38 @@ class fft_wrap_many:
         for st in self._bundle(bundle_id)["streams"]:
             runtime_api.stream_synchronize(st)
 
-    # -- hipFFT handle diagnostics -------------------------------------------
-    def _handle_diagnostics(self, *a, **k):
-        raise NotImplementedError("fft_wrap_many: handle/stream tokens and minimal-reference plan arrays expose "
-                                  "hipFFT handles; this backend has none")
+    # plan-array tokens (fft_wrap_many.h:1270-1296): the first / last plan handle and
+    # stream as integers, 0 for an empty array
+    def _array_token(self, array_id, which, kind):
+        arr = self._plan_array(array_id)
+        if not arr["plans"]:
+            return 0
+        if kind == "handle":
+            h = arr["plans"][which]._handle
+            return int(getattr(h, "value", h) or 0)
+        streams = arr["external_streams"] or arr["streams"]
+        return stream_handle(streams[which])
 
-    c2c_plan_array_first_handle_token = _handle_diagnostics
-    c2c_plan_array_last_handle_token = _handle_diagnostics
-    c2c_plan_array_first_stream_token = _handle_diagnostics
-    c2c_plan_array_last_stream_token = _handle_diagnostics
-    make_minimal_reference_c2c_plan_array_1D = _handle_diagnostics
+    def c2c_plan_array_first_handle_token(self, array_id) -> int:
+        return self._array_token(array_id, 0, "handle")
+
+    def c2c_plan_array_last_handle_token(self, array_id) -> int:
+        return self._array_token(array_id, -1, "handle")
+
+    def c2c_plan_array_first_stream_token(self, array_id) -> int:
+        return self._array_token(array_id, 0, "stream")
+
+    def c2c_plan_array_last_stream_token(self, array_id) -> int:
+        return self._array_token(array_id, -1, "stream")
+
+    # -- fftm's MPI-side diagnostics ---------------------------------------------
+    def _mpi_diagnostics(self, *a, **k):
+        raise NotImplementedError("fft_wrap_many: the minimal-reference plan arrays and the opt0 local plan "
+                                  "contexts are diagnostics of fftm's MPI transposes; C++ callers get them "
+                                  "from the reference's container over fftm::wrap::mi::fft")
+
+    make_minimal_reference_c2c_plan_array_1D_diagnostic = _mpi_diagnostics
+    make_reference_opt0_local_plan_context_1D = _mpi_diagnostics
 
     def __del__(self):
         try:

@@ -131,8 +131,11 @@ def test_c2c_plan_array(gpu):
         w.exec_c2c_plan_array_direction_dual_offsets_no_sync(aid, D.C2CB, offs, outs[:2], y, z)
     with pytest.raises(IndexError):
         w.c2c_plan_array_size(5)
+    assert w.c2c_plan_array_first_handle_token(aid) != w.c2c_plan_array_last_handle_token(aid)
+    assert w.c2c_plan_array_first_stream_token(aid) == streams[0].cuda_stream
+    assert w.c2c_plan_array_last_stream_token(aid) == streams[2].cuda_stream
     with pytest.raises(NotImplementedError):
-        w.c2c_plan_array_first_handle_token(aid)
+        w.make_minimal_reference_c2c_plan_array_1D_diagnostic(n, n, 1, n, n, 1, n, batch, offs)
     w.release()
 
 
