@@ -1888,7 +1888,7 @@ __global__ void k_count_diff( const uint32_t *a, const uint32_t *b, int64_t n, u
         atomicAdd( count, c );
 }
 
-// variants of the fused filter pass: tile shapes 0, 2 and 4, each in both tile orders
+// variants of the fused filter pass: tile shapes 0, 2 and 4, each in every tile order
 constexpr int FILTER_CANDIDATES[] = { 0, 2, 4, 8, 10, 12, 16, 18, 20, 24, 26, 28 };
 
 bool filter_variant_ok( int var )
@@ -2142,7 +2142,7 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             std::vector<pass_rt> one( 1, cur );
             double               best_ms = 1e300;
             int                  best    = cur.var;
-            // tile shapes 0..nvar-1, each in both tile orders
+            // tile shapes 0..nvar-1, each in every tile order
             // (plane passes: shape 0 the LDS kernel, 1 the register kernel; any-radix
             // passes have one shape and their own order)
             const int            nvar    = cur.generic ? 1 : ( cur.plane ? 2 : fixed_variants( cur.N ) );
