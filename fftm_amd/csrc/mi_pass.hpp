@@ -446,18 +446,24 @@ template <int KIND>
 __device__ __forceinline__ cpx<float> load_line_h16( const pass_params<float> &p, int64_t ia, int64_t ib, int64_t xa,
                                                      int64_t xb, bool va, bool vb, int n, int N )
 {
+    // unconditional loads (a column past the end reads element 0, which exists) and a
+    // select afterwards: a load under a branch makes the compiler wait for it at the
+    // join, one memory latency per element
     if constexpr ( KIND == KIND_C2C )
     {
-        if ( !va )
-            return mk<float>( 0, 0 );
-        return h16_load( p.in, ia + (int64_t)n * p.ies, p.xin, xa + (int64_t)n * p.xsn );
+        const cpx<float> v = h16_load( p.in, va ? ia + (int64_t)n * p.ies : 0, p.xin, va ? xa + (int64_t)n * p.xsn : 0 );
+        return va ? v : mk<float>( 0, 0 );
     }
     else
     {
         const int  h  = N / 2;
         const int  nn = n <= h ? n : N - n;
-        cpx<float> a  = va ? h16_load( p.in, ia + (int64_t)nn * p.ies, p.xin, xa + (int64_t)nn * p.xsn ) : mk<float>( 0, 0 );
-        cpx<float> b  = vb ? h16_load( p.in, ib + (int64_t)nn * p.ies, p.xin, xb + (int64_t)nn * p.xsn ) : mk<float>( 0, 0 );
+        cpx<float> a  = h16_load( p.in, va ? ia + (int64_t)nn * p.ies : 0, p.xin, va ? xa + (int64_t)nn * p.xsn : 0 );
+        cpx<float> b  = h16_load( p.in, vb ? ib + (int64_t)nn * p.ies : 0, p.xin, vb ? xb + (int64_t)nn * p.xsn : 0 );
+        if ( !va )
+            a = mk<float>( 0, 0 );
+        if ( !vb )
+            b = mk<float>( 0, 0 );
         if ( n > h )
         {
             a.y = -a.y;
@@ -618,10 +624,11 @@ struct pass_kernel
         else
             return t + TPC * b;
     }
-    // paired HBM accesses exist for real lines (two reals) and FP32 complex lines
-    static constexpr bool PAIR_LOAD  = IOM == IO_F32 && pair_stage<0>() &&
+    // paired HBM accesses exist for real lines (two reals) and FP32 complex lines; the
+    // real side of an R2C / C2R pass stays FP32 when its spectrum side is FP16-stored
+    static constexpr bool PAIR_LOAD  = !( IOM & IO_H16_IN ) && pair_stage<0>() &&
                                       ( KIND == KIND_R2C || ( KIND == KIND_C2C && sizeof( T ) == 4 ) );
-    static constexpr bool PAIR_STORE = IOM == IO_F32 && pair_stage<RP.n - 1>() &&
+    static constexpr bool PAIR_STORE = !( IOM & IO_H16_OUT ) && pair_stage<RP.n - 1>() &&
                                        ( KIND == KIND_C2R || ( KIND == KIND_C2C && sizeof( T ) == 4 ) );
 
     // largest component magnitude over the TPC threads that share a column: each thread
@@ -1138,27 +1145,28 @@ struct pass_kernel
                 xoff[l] = xi;
             }
             __syncthreads();
-            // every load is issued before the first LDS write
+            // every load is issued before the first LDS write, unconditionally (an item
+            // past the end or in a column past the end reads element 0, which exists) and
+            // selected afterwards, so that no load waits on a branch
             constexpr int ITEMS = 2 * TW * NH;
             constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
             C             zs[PER];
 #pragma unroll
             for ( int i = 0; i < PER; ++i )
             {
-                const int e = tid + i * NT;
-                zs[i]       = mk<T>( 0, 0 );
-                if ( ITEMS % NT == 0 || e < ITEMS )
-                {
-                    const int     line = e / NH, k = e % NH;
-                    const int64_t lo   = loff[line];
-                    if ( lo >= 0 )
-                    {
-                        if constexpr ( IOM & IO_H16_IN )
-                            zs[i] = h16_load( p.in, lo + (int64_t)k * p.ies, p.xin, xoff[line] + (int64_t)k * p.xsn );
-                        else
-                            zs[i] = static_cast<const C *>( p.in )[lo + (int64_t)k * p.ies];
-                    }
-                }
+                const int     e    = tid + i * NT;
+                const bool    in   = ITEMS % NT == 0 || e < ITEMS;
+                const int     ec   = in ? e : ITEMS - 1;
+                const int     line = ec / NH, k = ec % NH;
+                const int64_t lo   = loff[line];
+                const bool    ok   = in && lo >= 0;
+                const int64_t src  = ok ? lo + (int64_t)k * p.ies : 0;
+                C             z;
+                if constexpr ( IOM & IO_H16_IN )
+                    z = h16_load( p.in, src, p.xin, ok ? xoff[line] + (int64_t)k * p.xsn : 0 );
+                else
+                    z = static_cast<const C *>( p.in )[src];
+                zs[i] = ok ? z : mk<T>( 0, 0 );
             }
 #pragma unroll
             for ( int i = 0; i < PER; ++i )
