@@ -1253,6 +1253,73 @@ struct pass_kernel
                 loaded = true;
             }
         }
+        if constexpr ( LOADB && KIND == KIND_C2C && ( IOM & IO_H16_IN ) )
+        {
+            // strided FP16 lines whose exponents do not depend on the fastest batch index
+            // (the y pass of the 3D layout: element n of every column of one x plane has
+            // the exponent of z-line (x, n)).  With d0 >= TW a tile's columns span at most
+            // two such planes, so the tile stages their 2 N exponents in LDS, a few byte
+            // loads per thread instead of one per element.  Same values, same arithmetic.
+            if ( p.xs0 == 0 && p.d0 >= TW )
+            {
+                __shared__ int8_t xe[2 * N];
+                const int64_t     cf = x.tile * TW < p.ncols ? x.tile * TW : p.ncols - 1;
+                const int64_t     cl = cf + TW - 1 < p.ncols ? cf + TW - 1 : p.ncols - 1;
+                int64_t           xf = 0, xl = 0, yd = 0;
+                col_exponents( p, cf, xf, yd );
+                col_exponents( p, cl, xl, yd );
+                // the exponent loads are issued first and the data loads after them, all
+                // unconditionally (indices past the end read element 0, which exists)
+                constexpr int EP = ( 2 * N + NT - 1 ) / NT;
+                int8_t        er[EP];
+#pragma unroll
+                for ( int i = 0; i < EP; ++i )
+                {
+                    const int e  = tid + i * NT;
+                    const int ec = ( 2 * N ) % NT == 0 || e < 2 * N ? e : 0;
+                    er[i]        = p.xin[( ec < N ? xf : xl ) + (int64_t)( ec % N ) * p.xsn];
+                }
+                // the raw FP16 pair waits in the real component's register
+                const h16x2 *src = static_cast<const h16x2 *>( p.in );
+#pragma unroll
+                for ( int b = 0; b < NB0; ++b )
+                {
+                    const int  j  = jmap<0>( x.t, b );
+                    const bool ok = x.va && ( NBF0 % TPC == 0 || j < NBF0 );
+#pragma unroll
+                    for ( int r = 0; r < R0; ++r )
+                        v[b][r].x = __builtin_bit_cast( float, src[ok ? x.ia + (int64_t)( j + r * NBF0 ) * p.ies : 0] );
+                }
+#pragma unroll
+                for ( int i = 0; i < EP; ++i )
+                {
+                    const int e = tid + i * NT;
+                    if ( ( 2 * N ) % NT == 0 || e < 2 * N )
+                        xe[e] = er[i];
+                }
+                __syncthreads();
+                const int g = x.xa == xf ? 0 : N;
+#pragma unroll
+                for ( int b = 0; b < NB0; ++b )
+                {
+                    const int j = jmap<0>( x.t, b );
+                    if ( NBF0 % TPC == 0 || j < NBF0 )
+                    {
+#pragma unroll
+                        for ( int r = 0; r < R0; ++r )
+                        {
+                            const int   e = xe[g + j + r * NBF0];
+                            const h16x2 h = __builtin_bit_cast( h16x2, v[b][r].x );
+                            v[b][r]       = x.va ? mk<float>( __builtin_ldexpf( (float)h.x, e ),
+                                                              __builtin_ldexpf( (float)h.y, e ) )
+                                                 : mk<float>( 0, 0 );
+                        }
+                        dft_fixed<R0, INV>( v[b], p.tw, N );
+                    }
+                }
+                loaded = true;
+            }
+        }
         if ( !loaded )
         {
 #pragma unroll

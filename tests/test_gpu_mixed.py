@@ -132,3 +132,33 @@ def test_h16_variants_bitwise_and_autotune(gpu):
     f.backward(X, y)
     assert torch.equal(X, X0) and torch.equal(y, y0)
     f.close()
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128), (256, 128, 512)])
+def test_h16_line_scales_differ(gpu, dims):
+    """z-lines scaled by 2^s(x, y), s in [-5, 5]: every stored line of the first pass
+    gets its own exponent, so the y pass, whose column tiles straddle two x planes
+    (N/2+1 columns per plane), must give each column the exponents of its own plane.
+    Compared with the plain FP32 path at the FP16 forward tolerance: a column given
+    the other plane's exponents would be off by a power of two."""
+    import torch
+    print(f"case h16 line scales {dims}", flush=True)
+    nx, ny, _ = dims
+    x = oracle.fill_random_real_3d(*dims, 0x5EED, np.float32)
+    s = (3 * np.arange(nx)[:, None] + 5 * np.arange(ny)[None, :]) % 11 - 5
+    x = (x * np.exp2(s)[:, :, None]).astype(np.float32)
+    u = torch.from_numpy(x).cuda()
+    f, g = _ffts(dims), _ffts(dims, "fp32")
+    X, X32 = f.complex_array(), g.complex_array()
+    f.forward(u, X)
+    g.forward(u, X32)
+    l2 = float(torch.linalg.vector_norm(X - X32) / torch.linalg.vector_norm(X32))
+    print(f"[h16] line scales {dims}: fwd rel_l2 vs FP32 {l2:.3e}", flush=True)
+    assert l2 < TOL_FWD_L2
+    y = f.real_array()
+    f.backward(X, y)
+    rt = float(torch.linalg.vector_norm(y / np.prod(dims) - u) / torch.linalg.vector_norm(u))
+    print(f"[h16] line scales {dims}: round trip rel_l2 {rt:.3e}", flush=True)
+    assert rt < TOL_RT_L2
+    f.close()
+    g.close()
