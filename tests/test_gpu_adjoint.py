@@ -1,4 +1,4 @@
-"""Adjoints and spectral operators on the GPU (north-star differentiation target):
+"""Adjoints and spectral operators on the GPU (the differentiation workload):
 
 * dot-product identities <R2C u, G> = <u, R2C^T G>, <C2R X, v> = <X, C2R^T v>
   (the analytic adjoint), every layout, FP64;
