@@ -31,6 +31,7 @@ R2C, C2R, C2CF, C2CB = 1, 2, 3, 4
 F32, F64 = 0, 1
 F32_H16 = 2  # FP32 transforms, FP16-stored intermediate spectra
 FLAG_OVERWRITE_INPUT = 1
+FLAG_NO_PLANE_FUSION = 2
 
 LAYOUT_3D = 0
 LAYOUT_4D_PENCIL = 1

@@ -358,7 +358,6 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
             p.ys0            = r.ys0;
             p.ys1            = r.ys1;
             p.ys2            = r.ys2;
-            p.vec            = rr;
             return launch_pass_mixed_f32( p, (int)r.N, r.kind, r.inv, r.loadb, shape, r.iom, s );
         }
         if ( r.filter )
@@ -1021,7 +1020,7 @@ spec_geom make_spec_geom( const ffts_layout &L )
     int order[4] = { 0, 1, 2, 3 };
     std::sort( order, order + L.dim, [&]( int a, int b ) { return L.sc[a] > L.sc[b]; } );
     g.total     = 1;
-    double kmax2 = 0.0;
+    int64_t kmax2 = 0;
     for ( int m = 0; m < L.dim; ++m )
     {
         const int a = order[m];
@@ -1029,9 +1028,9 @@ spec_geom make_spec_geom( const ffts_layout &L )
         g.ext[m]    = a == L.dim - 1 ? L.n[a] / 2 + 1 : L.n[a];
         g.total *= g.ext[m];
         g.n[a] = L.n[a];
-        kmax2 += (double)( L.n[a] / 2 ) * (double)( L.n[a] / 2 );
+        kmax2 += ( L.n[a] / 2 ) * ( L.n[a] / 2 );
     }
-    g.kscale  = kmax2 > 0 ? 1.0 / std::sqrt( kmax2 ) : 0.0; // times nshells at use
+    g.kmax2   = kmax2 > 0 ? kmax2 : 1; // all-ones sizes: every bin has k = 0
     g.nshells = 1;
     return g;
 }
@@ -1469,6 +1468,8 @@ int fftm_mi_ffts_c2r_adjoint( fftm_mi_ffts h, const void *real_in, void *spectru
         return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_c2r_adjoint: null handle" );
     if ( !real_in || !spectrum_out || real_in == spectrum_out )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_c2r_adjoint: bad data pointers" );
+    if ( h->fwd_work && !h->work )
+        return fail( FFTM_MI_NO_WORKSPACE, "fftm_mi_ffts_c2r_adjoint: a work area is required (set_work_area)" );
     return run_passes( h->fwd, h->tb, h->prec, real_in, spectrum_out, h->work, h->stream, false, sync != 0,
                        "ffts_c2r_adjoint", 1 );
 }
@@ -1501,7 +1502,6 @@ int fftm_mi_ffts_spectral_multiply( fftm_mi_ffts h, const void *in, void *out, c
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_spectral_multiply: nshells must be in 1..1024" );
     spec_geom g = h->sg;
     g.nshells   = shell_coeffs ? nshells : 1;
-    g.kscale    = h->sg.kscale * (double)g.nshells;
     const int64_t blocks = std::min<int64_t>( ( g.total + 255 ) / 256, 8192 );
     if ( h->prec == FFTM_MI_F32 )
         hipLaunchKernelGGL( k_spectral_multiply<float>, dim3( (unsigned)blocks ), dim3( 256 ), 0, h->stream, g,
@@ -1534,7 +1534,6 @@ int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, cons
     }
     spec_geom g = h->sg;
     g.nshells   = nshells;
-    g.kscale    = h->sg.kscale * (double)nshells;
     const size_t lds = sizeof( double ) * (size_t)nshells;
     if ( h->prec == FFTM_MI_F32 )
     {
@@ -1602,7 +1601,7 @@ filter_params<T> make_filter_params( const fftm_mi_ffts_s &h, const fftm_mi_spec
     f.khi2 = op.khi * op.khi;
     f.scale          = op.scale;
     f.inv_two_sigma2 = op.kind == FFTM_MI_OP_GAUSSIAN ? 1.0 / ( 2.0 * op.sigma * op.sigma ) : 0.0;
-    f.kscale         = h.sg.kscale * (double)f.nshells;
+    f.kmax2          = h.sg.kmax2;
     f.coeffs         = static_cast<const T *>( op.coeffs );
     return f;
 }
@@ -1871,6 +1870,10 @@ std::string problem_key( const fftm_mi_ffts_s &h )
     if ( h.layout != FFTM_MI_LAYOUT_3D )
         o << "x" << h.n[3];
     o << ( h.h16 ? "_F32_H16" : ( h.prec == FFTM_MI_F32 ? "_F32" : "_F64" ) );
+    // creation flags change the pass list (plane fusion, in-place backward): records are
+    // per pass index, so a handle only takes records measured with the same flags
+    if ( h.flags )
+        o << "_FLAGS" << h.flags;
     return o.str();
 }
 

@@ -117,10 +117,28 @@ struct filter_params
     int64_t  nlen[4];  // full logical length of each memory axis; 0 marks the halved real axis
     double   scale;
     double   inv_two_sigma2;
-    double   kscale;   // S / kmax
+    int64_t  kmax2;    // sum over axes of (n_a / 2)^2 (radial shells)
     double   klo2, khi2; // OP_BANDPASS: squared band edges
     const T *coeffs;   // OP_SHELL coefficients (T[nshells])
 };
+
+// Radial shell of a bin with |k|^2 = k2: the largest s with s^2 kmax2 <= S^2 k2, i.e.
+// floor(S |k| / kmax) evaluated exactly, clamped to S - 1.  Integer arithmetic: on bins
+// where S |k| / kmax is an integer (|k|^2 = 3 on a 6^3 cube with S = 3) a floating
+// floor(sqrt(k2) * S / kmax) can land one shell low.  S <= 1024 and k2, kmax2 < 2^25 keep
+// every product below 2^46.
+__device__ __forceinline__ int shell_of( int64_t k2, int S, int64_t kmax2 )
+{
+    const int64_t rhs = (int64_t)S * S * k2;
+    int           s   = (int)sqrt( (double)rhs / (double)kmax2 ); // within one of the answer
+#pragma unroll
+    for ( int i = 0; i < 2; ++i )
+        s = ( s > 0 && (int64_t)s * s * kmax2 > rhs ) ? s - 1 : s;
+#pragma unroll
+    for ( int i = 0; i < 2; ++i )
+        s = ( (int64_t)( s + 1 ) * ( s + 1 ) * kmax2 <= rhs ) ? s + 1 : s;
+    return s < S - 1 ? s : S - 1;
+}
 
 // wavenumbers of the bin at spectrum offset off: returns sum_a k_a^2 and sets kd to the
 // wavenumber along memory axis f.dax (k = i on the halved axis, k = i <= n/2 ? i : i - n
@@ -160,14 +178,8 @@ __device__ __forceinline__ cpx<T> filter_h( const filter_params<T> &f, float k2,
     case OP_BANDPASS:
         return mk<T>( ( (double)k2 >= f.klo2 && (double)k2 <= f.khi2 ) ? (T)f.scale : (T)0, (T)0 );
     default:
-    {
-        // shell boundaries fall on exact products (e.g. |k|^2 = 12 m^2 on a cube):
-        // evaluate floor(sqrt(k2) * S / kmax) in FP64 with the same operation order as
-        // the radial multiply, so every bin lands in the same shell on every path
-        int s = (int)( sqrt( (double)k2 ) * f.kscale );
-        s     = s < f.nshells - 1 ? s : f.nshells - 1;
-        return mk<T>( (T)f.scale * f.coeffs[s], (T)0 );
-    }
+        // k2 is a sum of squared integers below 2^24, exact in FP32
+        return mk<T>( (T)f.scale * f.coeffs[shell_of( (int64_t)k2, f.nshells, f.kmax2 )], (T)0 );
     }
 }
 

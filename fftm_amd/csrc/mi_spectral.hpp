@@ -6,7 +6,7 @@
 // Wavenumbers follow the reference drivers: k = i <= n/2 ? i : i - n on full axes
 // (Nyquist positive), k = i on the halved real axis.  A radial shell filter uses
 // S shells of equal width over [0, kmax], kmax = sqrt(sum_a (n_a/2)^2):
-//   shell(k) = min(S - 1, floor(S * |k| / kmax)).
+//   shell(k) = min(S - 1, floor(S * |k| / kmax)), evaluated exactly (shell_of).
 #pragma once
 
 #include "mi_complex.hpp"
@@ -23,13 +23,13 @@ struct spec_geom
     int32_t ax[4];     // logical axis of each memory axis
     int64_t n[4];      // full logical lengths per logical axis
     int64_t total;     // number of complex bins
-    double  kscale;    // S / kmax
+    int64_t kmax2;     // sum over axes of (n_a / 2)^2
     int32_t nshells;
 };
 
 __device__ __forceinline__ int spec_shell( const spec_geom &g, int64_t i )
 {
-    double  k2 = 0.0;
+    int64_t k2 = 0;
     int64_t r  = i;
     for ( int m = g.nd - 1; m >= 0; --m )
     {
@@ -38,10 +38,9 @@ __device__ __forceinline__ int spec_shell( const spec_geom &g, int64_t i )
         const int     a   = g.ax[m];
         const int64_t na  = g.n[a];
         const int64_t k   = ( a == g.half_axis || idx <= na / 2 ) ? idx : idx - na;
-        k2 += (double)k * (double)k;
+        k2 += k * k;
     }
-    int s = (int)( sqrt( k2 ) * g.kscale );
-    return s < g.nshells - 1 ? s : g.nshells - 1;
+    return shell_of( k2, g.nshells, g.kmax2 );
 }
 
 // out[i] = H_i * in[i];  H_i = filter[i] (per bin, real) or coeffs[shell(i)]

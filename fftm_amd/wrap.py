@@ -241,8 +241,9 @@ class fft_wrap_many:
     and R2C/C2R plan bundles (lanes of 2D forward/inverse plan pairs with their own
     streams and work areas; :188-237, 310-378, 1815-1855).
 
-    The handle and stream "tokens" and the minimal-reference plan arrays, which expose
-    hipFFT handles for diagnostics (:1270-1365), raise ``NotImplementedError``."""
+    The plan-array handle and stream tokens (:1270-1296) return the plans' opaque handle
+    and stream values.  The minimal-reference plan arrays and the opt0 local plan contexts
+    (:1297-1365), diagnostics of fftm's MPI transposes, raise ``NotImplementedError``."""
 
     def __init__(self, real):
         self.real = real
@@ -689,8 +690,7 @@ class fft_wrap_many:
     # -- fftm's MPI-side diagnostics ---------------------------------------------
     def _mpi_diagnostics(self, *a, **k):
         raise NotImplementedError("fft_wrap_many: the minimal-reference plan arrays and the opt0 local plan "
-                                  "contexts are diagnostics of fftm's MPI transposes; C++ callers get them "
-                                  "from the reference's container over fftm::wrap::mi::fft")
+                                  "contexts are diagnostics of fftm's MPI transposes and are not mirrored")
 
     make_minimal_reference_c2c_plan_array_1D_diagnostic = _mpi_diagnostics
     make_reference_opt0_local_plan_context_1D = _mpi_diagnostics

@@ -201,8 +201,10 @@ int fftm_mi_ffts_destroy( fftm_mi_ffts h );
  * the real axis and w_k = 2 elsewhere, C2R(X) = Re sum_k w_k X_k e^{+i k.x}, hence
  *   R2C^T(G) = C2R(G / w)     (a Hermitian-weighted C2R)
  *   C2R^T(v) = w * R2C(v).
- * r2c_adjoint needs the work area like backward; both run the same passes as
- * backward / forward with the weights folded into the real-axis pass.
+ * They run the same passes as backward / forward with the weights folded into the
+ * real-axis pass, so each needs the work area when its counterpart does: r2c_adjoint
+ * like backward, c2r_adjoint like forward (FFTM_MI_F32_H16 handles keep the FP16
+ * spectrum there in both directions).
  */
 int fftm_mi_ffts_r2c_adjoint( fftm_mi_ffts h, const void *spectrum_in, void *real_out, int32_t sync );
 int fftm_mi_ffts_c2r_adjoint( fftm_mi_ffts h, const void *real_in, void *spectrum_out, int32_t sync );
@@ -211,7 +213,8 @@ int fftm_mi_ffts_c2r_adjoint( fftm_mi_ffts h, const void *real_in, void *spectru
  * out_k = H_k in_k on the handle's spectrum layout (in == out allowed).  H is real:
  * either per bin (filter: one T per complex bin, spectrum layout) or radial
  * (shell_coeffs[nshells]: H_k = shell_coeffs[shell(k)], shell(k) = min(S-1,
- * floor(S |k| / kmax)), kmax = sqrt(sum_a (n_a/2)^2), k = i <= n/2 ? i : i - n on full
+ * floor(S |k| / kmax)) evaluated exactly (the largest s with s^2 kmax^2 <= S^2 |k|^2),
+ * kmax = sqrt(sum_a (n_a/2)^2), k = i <= n/2 ? i : i - n on full
  * axes and k = i on the real axis).  Exactly one of filter / shell_coeffs is non-NULL.
  * The operator is self-adjoint, so it also serves as its own adjoint.
  */

@@ -1,8 +1,9 @@
 // Named-plan container over mi::fft<T,D>: the ffts subset of
 // fftm::wrap::fft_wrap_many<BaseFFTWrap,T> (source/external_wrap/fft_wrap_many.h:120-292,
-// 375-395, 1606-1680, 1753-1802) without the SCFD dependency.  The reference container
-// itself also compiles against mi::fft (it only needs the BaseFFTWrap surface of
-// mi_fft_wrap.h); this one exists because SCFD's arrays are not available here.
+// 375-395, 1606-1680, 1753-1802) without the SCFD dependency.  mi::fft offers the whole
+// BaseFFTWrap surface the reference container uses, but the reference container itself
+// includes SCFD's arrays (fft_wrap_many.h:15), which are absent here, so compiling it
+// against mi::fft is untested; this container exists for that reason.
 #ifndef FFTM_MI_FFT_WRAP_MANY_H
 #define FFTM_MI_FFT_WRAP_MANY_H
 

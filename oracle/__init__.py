@@ -428,16 +428,22 @@ def wavenumbers(n: int, halved: bool) -> np.ndarray:
 
 def shell_index_logical(dims, nshells: int) -> np.ndarray:
     """shell(k) = min(S-1, floor(S |k| / kmax)), kmax = sqrt(sum (n_a/2)^2), on the logical
-    half spectrum (last axis halved); returned as an int array of the logical half shape."""
+    half spectrum (last axis halved); returned as an int array of the logical half shape.
+    Evaluated exactly: the largest s with s^2 kmax^2 <= S^2 |k|^2 (integers throughout),
+    so bins where S |k| / kmax is an integer land in the upper shell."""
     d = len(dims)
-    ks = [wavenumbers(n, a == d - 1) for a, n in enumerate(dims)]
+    ks = [wavenumbers(n, a == d - 1).astype(np.int64) for a, n in enumerate(dims)]
     grids = np.meshgrid(*ks, indexing="ij")
-    k2 = np.zeros(grids[0].shape, dtype=np.float64)
+    k2 = np.zeros(grids[0].shape, dtype=np.int64)
     for g in grids:
-        k2 += g.astype(np.float64) ** 2
-    kmax2 = float(sum((n // 2) ** 2 for n in dims))
-    kscale = (1.0 / math.sqrt(kmax2)) * float(nshells)
-    s = np.floor(np.sqrt(k2) * kscale).astype(np.int64)
+        k2 += g * g
+    kmax2 = max(1, sum((n // 2) ** 2 for n in dims))
+    rhs = np.int64(nshells) * np.int64(nshells) * k2
+    s = np.floor(np.sqrt(rhs.astype(np.float64) / kmax2)).astype(np.int64)
+    for _ in range(2):
+        s = np.where((s > 0) & (s * s * kmax2 > rhs), s - 1, s)
+    for _ in range(2):
+        s = np.where((s + 1) * (s + 1) * kmax2 <= rhs, s + 1, s)
     return np.minimum(s, nshells - 1)
 
 

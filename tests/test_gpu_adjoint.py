@@ -106,6 +106,50 @@ def test_spectral_multiply_and_shell_gradient(gpu, kind, dims, real):
     del torch
 
 
+@pytest.mark.parametrize("kind,dims,S", [("3d", (6, 6, 6), 3), ("3d", (20, 20, 20), 10), ("3d", (30, 30, 30), 15),
+                                         ("pencil", (6, 6, 6, 6), 14), ("slab", (12, 12, 12, 12), 14),
+                                         ("xzwy", (6, 6, 6, 6), 14)])
+def test_shell_boundary_bins(gpu, kind, dims, S):
+    """Sizes with bins where S |k| / kmax is an integer (a floating floor(sqrt(k^2) S / kmax)
+    puts them one shell low): the shell multiply, the shell gradient and the fused shell
+    filter follow the exact shell index of the oracle (pinned against isqrt on the CPU)."""
+    import torch
+    d = len(dims)
+    ks = [oracle.wavenumbers(n, a == d - 1) for a, n in enumerate(dims)]
+    k2 = sum(gk.astype(np.float64) ** 2 for gk in np.meshgrid(*ks, indexing="ij"))
+    kmax2 = float(sum((n // 2) ** 2 for n in dims))
+    naive = [np.minimum(np.floor(np.sqrt(k2) * c).astype(np.int64), S - 1)
+             for c in ((1.0 / math.sqrt(kmax2)) * S, S / math.sqrt(kmax2))]
+    exact = oracle.shell_index_logical(dims, S)
+    assert any(np.count_nonzero(v != exact) > 0 for v in naive), "the case must contain boundary bins"
+    f = _make(kind, dims, "float64")
+    g = np.random.default_rng(5)
+    X = g.standard_normal(f.complex_shape()) + 1j * g.standard_normal(f.complex_shape())
+    coeffs = np.arange(1, S + 1, dtype=np.float64)          # distinct per shell
+    shell_mem = _logical_to_spec_mem(kind, exact)
+    out = f.complex_array()
+    f.spectral_multiply(to_gpu(X), out, shell_coeffs=to_gpu(coeffs))
+    assert np.array_equal(from_gpu(out), X * coeffs[shell_mem])
+    G = g.standard_normal(f.complex_shape()) + 1j * g.standard_normal(f.complex_shape())
+    grad = from_gpu(f.shell_gradient(to_gpu(G), to_gpu(X), S))
+    ref = np.bincount(shell_mem.reshape(-1), weights=(G.real * X.real + G.imag * X.imag).reshape(-1), minlength=S)
+    assert np.max(np.abs(grad - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # the analytic shell operator of the spectral-apply kernel and the fused filter pass
+    f.spectral_apply(to_gpu(X), out, "shell", coeffs=to_gpu(coeffs))
+    assert np.array_equal(from_gpu(out), X * coeffs[shell_mem])
+    u = g.uniform(-1, 1, f.real_shape())
+    y = f.real_array()
+    f.filter(to_gpu(u), y, "shell", scale=1.0, coeffs=to_gpu(coeffs))
+    Fu = f.complex_array()
+    f.forward(to_gpu(u), Fu)
+    Fu_h = from_gpu(Fu) * coeffs[shell_mem]
+    y2 = f.real_array()
+    f.backward(to_gpu(Fu_h), y2)
+    assert np.max(np.abs(from_gpu(y) - from_gpu(y2))) <= 1e-12 * np.max(np.abs(from_gpu(y2)))
+    f.close()
+    del torch
+
+
 def _loss_setup(n, S=32, seed=3, real="float64"):
     import torch
     f = _make("3d", (n, n, n), real)

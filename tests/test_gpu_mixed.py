@@ -162,3 +162,79 @@ def test_h16_line_scales_differ(gpu, dims):
     assert rt < TOL_RT_L2
     f.close()
     g.close()
+
+
+def test_h16_paired_end_passes_bitwise(gpu):
+    """The FP16-storage R2C and C2R passes move their FP32 real side as 8-byte pairs when
+    the real array is 16-byte aligned and element-wise otherwise: a real array at an
+    8-byte offset turns pairing off and must give bitwise the same spectrum and output."""
+    import torch
+    dims = (128, 128, 256)
+    n = int(np.prod(dims))
+    print(f"case h16 paired end passes {dims}", flush=True)
+    x = oracle.fill_random_real_3d(*dims, 0x1234ABCDEF987654, np.float32)
+    f = _ffts(dims)
+    a = torch.from_numpy(x).cuda()
+    buf = torch.zeros(n + 2, dtype=torch.float32, device="cuda")
+    off = buf[2:].view(dims)
+    off.copy_(a)
+    assert a.data_ptr() % 16 == 0 and off.data_ptr() % 16 == 8
+    X1, X2 = f.complex_array(), f.complex_array()
+    f.forward(a, X1)
+    f.forward(off, X2)
+    assert torch.equal(X1, X2)
+    y1 = f.real_array()
+    f.backward(X1, y1)
+    ybuf = torch.zeros(n + 2, dtype=torch.float32, device="cuda")
+    y2 = ybuf[2:].view(dims)
+    f.backward(X1, y2)
+    assert torch.equal(y1, y2)
+    f.close()
+
+
+def test_h16_adjoints_match_fp32(gpu):
+    """The adjoints of an FP16-storage handle (the same passes as forward / backward with
+    the real-axis weights) against the plain FP32 handle's, at the FP16 tolerance."""
+    import torch
+    dims = (128, 128, 128)
+    print(f"case h16 adjoints {dims}", flush=True)
+    f16, f32 = _ffts(dims), _ffts(dims, "fp32")
+    g = torch.Generator(device="cpu").manual_seed(11)
+    v = (torch.rand(dims, generator=g) * 2 - 1).cuda()
+    cs = f32.complex_shape()
+    G = torch.complex(torch.randn(cs, generator=g), torch.randn(cs, generator=g)).cuda()
+    A16, A32 = f16.complex_array(), f32.complex_array()
+    f16.c2r_adjoint(v, A16)
+    f32.c2r_adjoint(v, A32)
+    l2, _ = oracle.rel_errors(from_gpu(A16), from_gpu(A32).astype(np.complex128))
+    print(f"[h16] c2r_adjoint rel_l2 vs fp32 = {l2:.3e}", flush=True)
+    assert l2 < TOL_FWD_L2
+    B16, B32 = f16.real_array(), f32.real_array()
+    f16.r2c_adjoint(G, B16)
+    f32.r2c_adjoint(G, B32)
+    l2, _ = oracle.rel_errors(from_gpu(B16), from_gpu(B32).astype(np.float64))
+    print(f"[h16] r2c_adjoint rel_l2 vs fp32 = {l2:.3e}", flush=True)
+    assert l2 < TOL_FWD_L2
+    f16.close()
+    f32.close()
+
+
+def test_h16_adjoint_without_work_area_is_refused(gpu):
+    """An FP16-storage handle keeps its spectrum in the work area in both directions:
+    c2r_adjoint, like forward, refuses to run before one is bound (no launch happens)."""
+    import ctypes
+    import torch
+    from fftm_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    n = (ctypes.c_int64 * 3)(128, 128, 128)
+    assert L.fftm_mi_ffts_create(0, n, _lib.F32_H16, 0, ctypes.byref(h)) == 0
+    try:
+        u = torch.zeros((128, 128, 128), dtype=torch.float32, device="cuda")
+        X = torch.zeros((128, 128, 65), dtype=torch.complex64, device="cuda")
+        assert L.fftm_mi_ffts_c2r_adjoint(h, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(X.data_ptr()), 1) == \
+            _lib.NO_WORKSPACE
+        assert L.fftm_mi_ffts_forward(h, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(X.data_ptr()), 1) == \
+            _lib.NO_WORKSPACE
+    finally:
+        L.fftm_mi_ffts_destroy(h)

@@ -95,3 +95,42 @@ def test_autotune_cache_round_trip(gpu, tmp_path, dt, dims):
     assert torch.equal(g3, g0)
     h.close()
     f.close()
+
+
+def test_cache_records_are_keyed_by_creation_flags(gpu, tmp_path):
+    """Tuning records are per pass index, and creation flags change the pass list (a plane-
+    fused 32^4 pencil handle has 3 forward passes, an unfused one 4): a cache written by the
+    fused handle applies nothing to an unfused handle of the same size, and everything to
+    another fused one."""
+    import ctypes
+    import torch
+    import fftm_amd
+    from fftm_amd import _lib
+    L = _lib.lib()
+    dims = (32, 32, 32, 32)
+    print(f"case tuning flags {dims}", flush=True)
+    f = fftm_amd.ffts("float32", strategy_4d=fftm_amd.strategy_4d_pencil_pencil())
+    f.init(*dims, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key=""))
+    u = torch.rand(f.real_shape(), dtype=f.real, device="cuda") * 2 - 1
+    cache = str(tmp_path / "flags.cache")
+    chosen = f.autotune(warmup=1, iterations=2, cache_path=cache)
+    nf, nb = f.num_passes()
+    assert len(chosen) == nf + nb
+    del u
+    n = _lib.i64_array(dims)
+    for flags, expect in ((_lib.FLAG_NO_PLANE_FUSION, 0), (0, None)):
+        h = ctypes.c_void_p()
+        _lib.check(L.fftm_mi_ffts_create(_lib.LAYOUT_4D_PENCIL, n, _lib.F32, flags, ctypes.byref(h)), "create")
+        try:
+            fp, bp = ctypes.c_int32(), ctypes.c_int32()
+            _lib.check(L.fftm_mi_ffts_num_passes(h, ctypes.byref(fp), ctypes.byref(bp)), "num_passes")
+            applied = ctypes.c_int32(-1)
+            _lib.check(L.fftm_mi_ffts_load_tuning(h, cache.encode(), ctypes.byref(applied)), "load_tuning")
+            if flags:
+                assert fp.value == nf + 1, "the unfused handle has one more forward pass"
+                assert applied.value == 0
+            else:
+                assert applied.value >= nf + nb
+        finally:
+            L.fftm_mi_ffts_destroy(h)
+    f.close()

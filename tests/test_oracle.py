@@ -330,3 +330,80 @@ def test_oracle_spectral_operators_closed_forms():
     bp = oracle.filter_logical(src, oracle.spectral_operator_logical(dims, "bandpass", 1.0 / N, band=(4.0, 7.0)))
     be = np.cos(5 * Y + 4 * Z)  # |k| = 6.4 only
     assert np.abs(bp - be).max() < 1e-12
+
+
+# ---------------------------------------------------------------- wavenumbers, radial shells
+
+def test_wavenumber_map_vs_numpy_fftfreq():
+    """k = i <= n/2 ? i : i - n with the Nyquist bin positive (versioned_ffts_3d_test_driver.h:
+    140-142): numpy's fftfreq(n) * n, with its negative Nyquist flipped; the halved axis is
+    0..n/2."""
+    for n in (1, 2, 3, 7, 8, 12, 33, 64):
+        ref = np.rint(np.fft.fftfreq(n) * n).astype(np.int64)
+        if n % 2 == 0:
+            ref[n // 2] = n // 2
+        assert np.array_equal(oracle.wavenumbers(n, False), ref)
+        assert np.array_equal(oracle.wavenumbers(n, True), np.arange(n // 2 + 1))
+
+
+def _shell_isqrt(dims, nshells):
+    """Radial shell index from Python's exact integer square root: for integers a, b > 0,
+    floor(sqrt(a / b)) = isqrt(a // b) (s^2 <= a / b iff s^2 <= floor(a / b)), so
+    floor(S |k| / kmax) = isqrt(S^2 |k|^2 // kmax^2), clamped to S - 1, with
+    kmax^2 = sum (n_a / 2)^2 and |k|^2 over the reference wavenumber map."""
+    d = len(dims)
+    ks = []
+    for a, n in enumerate(dims):
+        if a == d - 1:
+            ks.append(list(range(n // 2 + 1)))
+        else:
+            ks.append([i if i <= n // 2 else i - n for i in range(n)])
+    kmax2 = sum((n // 2) ** 2 for n in dims)
+    grids = np.meshgrid(*[np.array(k, dtype=np.int64) for k in ks], indexing="ij")
+    k2 = sum(g * g for g in grids)
+    if kmax2 == 0:  # all-ones sizes: the only bin is k = 0
+        return np.zeros(k2.shape, dtype=np.int64)
+    table = {int(v): min(math.isqrt(nshells * nshells * int(v) // kmax2), nshells - 1)
+             for v in np.unique(k2)}
+    return np.vectorize(table.__getitem__, otypes=[np.int64])(k2)
+
+
+def test_shell_index_hand_enumerated():
+    """A 4x4x4 cube with 4 shells by hand: kmax^2 = 12, shell = floor(4 |k| / sqrt 12),
+    so |k|^2 = 0 -> 0, 1 -> 1, 2 -> 1, 3 -> 2 (exactly 4*sqrt3/sqrt12 = 2, the boundary),
+    4 -> 2, 5 -> 2, 6 -> 2, 8 -> 3, 9 -> 3, 12 -> 3 (clamped from 4).  The 6^3 cube with 3
+    shells has a boundary bin: kmax^2 = 27, |k|^2 = 3 gives 3 sqrt3 / sqrt27 = 1 exactly,
+    shell 1 (a floating floor(sqrt(3) * 3 / sqrt(27)) gives 0)."""
+    sh = oracle.shell_index_logical((4, 4, 4), 4)
+    expect = {0: 0, 1: 1, 2: 1, 3: 2, 4: 2, 5: 2, 6: 2, 8: 3, 9: 3, 12: 3}
+    k = [np.array([0, 1, 2, -1]), np.array([0, 1, 2, -1]), np.array([0, 1, 2])]
+    seen = set()
+    for i in range(4):
+        for j in range(4):
+            for l in range(3):
+                k2 = int(k[0][i] ** 2 + k[1][j] ** 2 + k[2][l] ** 2)
+                assert sh[i, j, l] == expect[k2], (i, j, l, k2)
+                seen.add(k2)
+    assert seen == set(expect)
+    # 4D, 2x2x2x2 with 2 shells: kmax^2 = 4, shell = floor(2 |k| / 2) = floor(|k|) clamped to 1
+    sh4 = oracle.shell_index_logical((2, 2, 2, 2), 2)
+    assert sh4[0, 0, 0, 0] == 0 and sh4[1, 0, 0, 0] == 1 and sh4[1, 1, 1, 1] == 1
+    sh6 = oracle.shell_index_logical((6, 6, 6), 3)
+    assert sh6[1, 1, 1] == 1 and sh6[1, 0, 0] == 0 and sh6[2, 2, 2] == 2 and sh6[3, 3, 3] == 2
+
+
+@pytest.mark.parametrize("dims,nshells", [
+    ((4, 4, 4), 4), ((32, 32, 32), 32), ((64, 64, 64), 32), ((256, 256, 256), 32),
+    ((12, 10, 8), 32), ((12, 10, 8), 7), ((16, 8, 10, 12), 32), ((8, 6, 10, 12), 32),
+    ((32, 32, 32, 32), 32), ((20, 20, 20), 10), ((6, 6, 6), 3), ((30, 30, 30), 15),
+    ((10, 10, 10, 10), 5), ((1, 1, 1), 4),
+])
+def test_shell_index_exact_integer_arithmetic(dims, nshells):
+    """oracle.shell_index_logical equals the isqrt answer at every bin, boundary bins (where
+    S^2 |k|^2 / kmax^2 is a perfect square) included; (20, 20, 20) with 10 shells has 20 of
+    them, (30, 30, 30) with 15 has 68."""
+    got = oracle.shell_index_logical(dims, nshells)
+    exp = _shell_isqrt(dims, nshells)
+    assert got.shape == exp.shape
+    bad = int(np.count_nonzero(got != exp))
+    assert bad == 0, f"{bad} bins differ"
