@@ -13,11 +13,13 @@ Prints ONE JSON line.  ``value`` is GFLOP/s with the 2.5 N log2 N per-R2C conven
     average duration (HIP events on the transform's stream over the timed region),
     against 8 TB/s;
   * ``hbm_gbps_alg``: B_alg = 2 [(4N + 8N_c) + (d-1) 16 N_c] bytes per pair / pair time;
-  * errors of the timed transform against the FP64 oracle (forward spectrum) and the
-    round trip;
+  * max and RMS relative errors of the timed transform (forward spectrum) against an
+    FP64 transform of the same input bytes (pocketfft on the host; this library's FP64
+    path on the GPU at 1024^3), and of the round trip;
   * the 64^4 pencil-layout 4D pair (GFLOP/s, HBM GB/s);
-  * ``cpu_baseline``: the FP64 CPU oracle timed on this host's cores on a bounded
-    sample of the same 512^3 pair.
+  * ``cpu_baseline``: the reference's CPU benchmark protocol (FFTW harness: 3 warmup +
+    10 timed pairs, forward and backward timed apart) with pocketfft standing in for the
+    absent FFTW, on this host's cores, on the same 512^3 field.
 """
 
 from __future__ import annotations
@@ -61,6 +63,46 @@ def _traffic_summary(kernel_key):
         return None
 
 
+def fill_field(u, dims, pencil=False):
+    """The benchmark harness's splitmix64 field (fill_random_real_{3d,4d}_functor,
+    fft_benchmark_common.h:201-257) written by the library's device fill into u, whose
+    memory layout is the ffts real layout: [x][y][z] (3D), [x][y][z][w] (4D slab) or
+    [w][z][y][x] (4D pencil)."""
+    import fftm_amd
+    nd = len(dims)
+    strides = [1] * nd
+    if pencil:       # memory [w][z][y][x]: x fastest
+        for a in range(1, nd):
+            strides[a] = strides[a - 1] * dims[a - 1]
+    else:
+        for a in range(nd - 2, -1, -1):
+            strides[a] = strides[a + 1] * dims[a + 1]
+    fftm_amd.fill_random_real(u, dims, SEED, strides, sync=True)
+    return u
+
+
+def fp64_spectrum(x_mem, pencil=False):
+    """FP64 reference spectrum (pocketfft via scipy.fft, all host threads) of a real field in
+    an ffts memory layout, returned in that layout's spectrum order: [x][y][z/2+1] for
+    [x][y][z]; for the 4D pencil layout real [w][z][y][x] -> spectrum [y][z][w/2+1][x]."""
+    import scipy.fft as sfft
+    w = _host_threads()
+    x = np.asarray(x_mem, dtype=np.float64)
+    if not pencil:
+        return sfft.rfftn(x, workers=w)
+    logical = x.transpose(3, 2, 1, 0)                       # u[x][y][z][w]
+    return np.ascontiguousarray(sfft.rfftn(logical, workers=w).transpose(1, 2, 3, 0))
+
+
+def rel_errors(actual, expected):
+    """(||a - e||_2 / ||e||_2, max|a - e| / max|e|)."""
+    a = np.asarray(actual).astype(np.complex128)
+    e = np.asarray(expected).astype(np.complex128)
+    d = np.abs(a - e)
+    ae = np.abs(e)
+    return float(np.linalg.norm(d) / np.linalg.norm(ae)), float(d.max() / ae.max())
+
+
 TUNING_CACHE = None  # --tuning-cache PATH
 
 
@@ -86,10 +128,10 @@ def tune_handle(f, warmup, iterations):
 
 
 def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None):
-    """Time the forward + backward pair.  check: True (FP64 CPU oracle on the same field),
-    "gpu64" (this library's FP64 transform of the same field as the reference, for sizes
-    where the CPU oracle takes too long) or False.  u_dev: a device input (else the
-    splitmix64 field)."""
+    """Time the forward + backward pair.  check: True (FP64 pocketfft on the host, same
+    input bytes), "gpu64" (this library's FP64 transform of the same field, for sizes where
+    the host transform takes too long) or False.  u_dev: a device input (else the
+    splitmix64 field, filled on the device)."""
     import torch
     import fftm_amd
     from fftm_amd import _lib
@@ -109,18 +151,14 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
     # --tuning-cache, a cache that already holds this problem is applied instead
     variants = tune_handle(f, warmup=2, iterations=7) if tune else f.pass_variants()
 
-    # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor), resident in HBM
-    import oracle
+    # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor) generated on the
+    # device by the library's fill kernel, resident in HBM
+    pencil = len(dims) == 4 and strategy.family.value == "pencil_pencil"
     if u_dev is not None:
         u = u_dev
-        x_host = None
     else:
-        if len(dims) == 3:
-            x_host = oracle.fill_random_real_3d(*dims, SEED, dt)
-        else:
-            x_host = oracle.to_memory(oracle.fill_random_real_4d_logical(*dims, SEED, dt),
-                                      oracle.A0123 if strategy.family.value == "pencil_pencil" else oracle.A3210)
-        u = torch.from_numpy(x_host).cuda()
+        u = f.real_array()
+        fill_field(u, dims, pencil)
     X = f.complex_array()
     y = f.real_array()
     nf, nb = f.num_passes()
@@ -219,24 +257,20 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
     elif check:
         pair()
         torch.cuda.synchronize()
-        if x_host is None:
-            x_host = u.cpu().numpy()
+        x_host = u.cpu().numpy().astype(np.float64)
         yr = y.cpu().numpy().astype(np.float64) / float(np.prod(dims))
-        d = yr - x_host.astype(np.float64)
+        d = yr - x_host
         res["roundtrip_rms"] = float(np.sqrt(np.mean(d * d)))
         res["roundtrip_max"] = float(np.max(np.abs(d)))
-        res["roundtrip_rel_rms"] = float(np.linalg.norm(d) / np.linalg.norm(x_host.astype(np.float64)))
+        res["roundtrip_rel_rms"] = float(np.linalg.norm(d) / np.linalg.norm(x_host))
+        del yr, d
         f.forward(u, X)
         torch.cuda.synchronize()
-        Xg = X.cpu().numpy()
-        if len(dims) == 3:
-            Xo = oracle.ffts3d_forward(x_host.astype(np.float64))
-        else:
-            Xo = oracle.ffts4d_forward(x_host.astype(np.float64), "pencil", *dims)
-        rms, mx = oracle.rel_errors(Xg, Xo)
+        rms, mx = rel_errors(X.cpu().numpy(), fp64_spectrum(x_host, pencil))
         res["fwd_rel_l2_vs_fp64"] = rms
         res["fwd_rel_max_vs_fp64"] = mx
-        del Xo, Xg
+        res["reference"] = "pocketfft (scipy.fft) FP64 on the host, same input bytes"
+        del x_host
     res["bytes_per_pair"] = int(sum(pbytes.values()))
     f.close()
     del u, X, y
@@ -381,33 +415,30 @@ def _graph_pair_ms(f, u, X, y, steps):
 
 def run_config1(steps, warmup):
     """128^3 round trip on the splitmix64 field in FP64 (the reference's CPU-oracle case)
-    and FP32, with errors against the FP64 C oracle.  Plain calls, and the same pair
+    and FP32, with errors against FP64 pocketfft of the same input.  Plain calls, and the same pair
     replayed from a HIP graph (the errors are those of the replayed outputs)."""
     import torch
     import fftm_amd
-    import oracle
     dims = (128, 128, 128)
-    out = {"workload": "ffts 3D R2C + C2R round trip 128^3, splitmix64 uniform(-1,1) field, normalised by 1/N"}
-    Xo = None
+    out = {"workload": "ffts 3D R2C + C2R round trip 128^3, splitmix64 uniform(-1,1) field, normalised by 1/N",
+           "reference": "pocketfft (scipy.fft) FP64 on the host, same input bytes"}
     for dt in ("float64", "float32"):
-        npdt = np.float64 if dt == "float64" else np.float32
-        x = oracle.fill_random_real_3d(*dims, SEED, npdt)
-        if Xo is None:
-            Xo = oracle.ffts3d_forward(x.astype(np.float64))
         f = fftm_amd.ffts(dt)
         f.init(*dims, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", sync=False))
         f.set_stream(torch.cuda.current_stream())
         tune_handle(f, warmup=2, iterations=7)
-        u = torch.from_numpy(x).cuda()
+        u = fill_field(f.real_array(), dims)
+        x = u.cpu().numpy()
+        Xo = fp64_spectrum(x)
         X, y = f.complex_array(), f.real_array()
         t = _timed(lambda: (f.forward(u, X), f.backward(X, y)), steps, warmup)
         X.zero_()
         y.zero_()
         tg = _graph_pair_ms(f, u, X, y, max(steps, 20))
-        rms, mx = oracle.rel_errors(X.cpu().numpy(), Xo)
+        rms, mx = rel_errors(X.cpu().numpy(), Xo)
         d = y.cpu().numpy().astype(np.float64) / float(np.prod(dims)) - x.astype(np.float64)
         out[dt] = {"ms_per_pair": round(t * 1e3, 4), "ms_per_pair_hip_graph": round(tg, 4),
-                   "fwd_rel_l2_vs_fp64_oracle": rms, "fwd_rel_max_vs_fp64_oracle": mx,
+                   "fwd_rel_l2_vs_fp64": rms, "fwd_rel_max_vs_fp64": mx,
                    "roundtrip_rms": float(np.sqrt(np.mean(d * d)))}
         f.close()
     return out
@@ -490,25 +521,76 @@ def run_adjoint(n, steps, warmup, nshells=32):
             "ms_loss_forward": round(t_fwd * 1e3, 4), "ms_loss_and_gradients": round(t_all * 1e3, 4)}
 
 
-def run_cpu_baseline(dims, pairs=2, warmup=1):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    """Threads this process may use: the affinity mask, capped by OMP_NUM_THREADS when set
+    (the GPU box grants a 16-thread share of a larger host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def run_cpu_baseline(dims, pairs=10, warmup=3):
+    """The reference's CPU baseline protocol (cpu_fftw_mpi/fftw_mpi_benchmark.cpp:249-276,
+    621-698): out-of-place R2C then C2R of one rank's whole field, `warmup` untimed pairs, then
+    `pairs` timed pairs with forward and backward timed separately; mean / population stddev /
+    median / min / max / p95 per pair, and the round-trip relative L2 error after 1/N.
+    FFTW is not installed here, so the transform is pocketfft (scipy.fft.rfftn / irfftn with
+    one worker per host thread), in FP64 (the reference harness's precision, the `value`) and
+    FP32.  The input is the splitmix64 field of the GPU measurement (same seed)."""
+    import scipy
+    import scipy.fft as sfft
     import oracle
-    L = oracle.lib()
-    threads = int(L.fo_max_threads())
-    x = oracle.fill_random_real_3d(*dims, SEED, np.float64)
+    threads = _host_threads()
     N = int(np.prod(dims))
-    for _ in range(warmup):
-        X = oracle.ffts3d_forward(x)
-        oracle.ffts3d_backward(X, dims[2])
-    t0 = time.perf_counter()
-    for _ in range(pairs):
-        X = oracle.ffts3d_forward(x)
-        oracle.ffts3d_backward(X, dims[2])
-    dt = (time.perf_counter() - t0) / pairs
-    return {"value": _flops_pair(N) / dt / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"{warmup} warmup + {pairs} timed R2C+C2R pairs of the same {dims[0]}x{dims[1]}x{dims[2]} "
-                      f"splitmix64 field in FP64 (oracle/fft_oracle.c, OpenMP {threads} threads); "
-                      f"{dt * 1e3:.1f} ms per pair",
-            "ms_per_pair": dt * 1e3}
+    out = {}
+    for dt in (np.float64, np.float32):
+        x = oracle.fill_random_real_3d(*dims, SEED, dt)
+        X = sfft.rfftn(x, workers=threads)           # plan creation outside the timed loop
+        y = sfft.irfftn(X, s=dims, workers=threads)
+        fwd, bwd = [], []
+        for k in range(warmup + pairs):
+            t0 = time.perf_counter()
+            X = sfft.rfftn(x, workers=threads)
+            t1 = time.perf_counter()
+            y = sfft.irfftn(X, s=dims, workers=threads)   # pocketfft's irfftn includes the 1/N
+            t2 = time.perf_counter()
+            if k >= warmup:
+                fwd.append((t1 - t0) * 1e3)
+                bwd.append((t2 - t1) * 1e3)
+        err = float(np.linalg.norm((y - x).astype(np.float64)) / np.linalg.norm(x.astype(np.float64)))
+        pair = [a + b for a, b in zip(fwd, bwd)]
+        stats = timing_stats(pair)
+        stats["max"] = round(float(np.max(pair)), 4)
+        out["f64" if dt == np.float64 else "f32"] = {
+            "gflops": round(_flops_pair(N) / (stats["mean"] * 1e-3) / 1e9, 2),
+            "pair_ms": stats, "forward_ms": timing_stats(fwd), "backward_ms": timing_stats(bwd),
+            "roundtrip_rel_l2": err}
+        del x, X, y
+    f64 = out["f64"]
+    return {"value": f64["gflops"], "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"pocketfft (scipy {scipy.__version__} scipy.fft.rfftn/irfftn, workers={threads}; FFTW "
+                      f"unavailable) on the same {dims[0]}x{dims[1]}x{dims[2]} splitmix64 field, FP64 and FP32, "
+                      f"{warmup} warmup + {pairs} timed R2C+C2R pairs each (the FFTW harness protocol); value = "
+                      f"FP64 mean {f64['pair_ms']['mean']:.1f} ms per pair",
+            "cpu_model": _cpu_model(), "ms_per_pair": f64["pair_ms"]["mean"], **out}
+
+
+def _progress(msg):
+    """One progress line per finished leg, on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -525,7 +607,8 @@ def main():
                     help="autotune cache file: applied where it holds the problem, else written by the tuner")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 256^3, FP16-storage and Poisson secondary measurements")
-    ap.add_argument("--big", action="store_true", help="also run the 1024^3 FP32 vs FP16-storage comparison")
+    ap.add_argument("--no-big", dest="big", action="store_false",
+                    help="skip the 1024^3 FP32 vs FP16-storage comparison (the mixed-precision workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -561,6 +644,7 @@ def main():
     r3 = run_gpu(dims, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check,
                  not args.no_tune)
     pair_s = r3["pair_s"]
+    _progress(f"{n}^3 FP32 pair {pair_s * 1e3:.4f} ms")
     gflops = _flops_pair(N) / pair_s / 1e9
     balg = _b_alg(dims, 4)
     # dominant pass kernel: its average duration over the timed region (HIP events)
@@ -597,13 +681,14 @@ def main():
                      "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "avg_ms": round(dom_ms, 4)},
     }
     if not args.no_check:
-        out["errors_vs_fp64_oracle"] = {k: r3[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64",
+        out["errors_vs_fp64"] = {k: r3[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64",
                                                            "roundtrip_rms", "roundtrip_max")}
     if not args.no_4d:
         d4 = (64, 64, 64, 64)
         r4 = run_gpu(d4, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, not args.no_check,
                      not args.no_tune)
         N4 = 64 ** 4
+        _progress(f"64^4 pencil pair {r4['api_pair_s'] * 1e3:.4f} ms")
         out["secondary_64^4"] = {
             "workload": "ffts 4D pencil layout R2C+C2R pair 64^4 float32 (real [w][z][y][x] -> [y][z][w/2+1][x])",
             # plain API pairs; events between the ~25 us passes add launch gaps
@@ -616,7 +701,7 @@ def main():
             "pass_variants": r4["variants"],
         }
         if not args.no_check:
-            out["secondary_64^4"]["errors_vs_fp64_oracle"] = {
+            out["secondary_64^4"]["errors_vs_fp64"] = {
                 k: r4[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64", "roundtrip_rms", "roundtrip_max")}
     if not args.no_extra:
         r256 = run_gpu((256, 256, 256), fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, False,
@@ -642,6 +727,7 @@ def main():
         if "fwd_rel_l2_vs_fp64" in rm:
             mp["meets_1e-5_rms"] = bool(rm["fwd_rel_l2_vs_fp64"] <= 1e-5)
         out["mixed_precision"] = mp
+        _progress(f"256^3 and {n}^3 FP16-storage legs done")
         if args.big:
             # 1024^3: uniform(-1,1) + k^-5/3 field; FP64 reference from this library on the GPU
             import torch
@@ -655,6 +741,7 @@ def main():
                                 "hbm_gbps": round(rb["bytes_per_pair"] / rb["pair_s"] / 1e9, 1),
                                 **{k: rb[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64",
                                                       "roundtrip_rel_rms") if k in rb}}
+            _progress("1024^3 FP32 and FP16-storage legs done")
             big["speedup_fp16_vs_fp32"] = round(big["fp32"]["ms_per_pair"] / big["fp16"]["ms_per_pair"], 3)
             big["input"] = "uniform(-1,1) + k^(-5/3) random-phase field (half the RMS), 1024^3 FP32"
             big["reference"] = "this library's FP64 transform of the same field on the GPU"
@@ -662,12 +749,15 @@ def main():
             del ub
             torch.cuda.empty_cache()
         out["poisson_512^3"] = run_poisson(512, args.steps, args.warmup)
+        _progress("Poisson 512^3 done")
         out["config1_128^3"] = run_config1(args.steps, args.warmup)
         out["gaussian_4d"] = run_gaussian_4d(max(3, args.steps // 2), 2)
+        _progress("128^3 round trip and 4D Gaussian filter done")
         out["adjoint_256^3"] = run_adjoint(256, max(3, args.steps // 2), 2)
     else:
         out["mixed_precision"] = None
     if not args.no_cpu:
+        _progress("GPU legs done; CPU baseline (pocketfft, FP64 and FP32)")
         out["cpu_baseline"] = run_cpu_baseline(dims)
     print(json.dumps(out), flush=True)
 

@@ -7,7 +7,7 @@ of ``include/fftm_mi.h``; this package is the host-side mirror of the reference'
 
 from ._lib import FftmMiError, LIB_PATH  # noqa: F401
 from .wrap import (LogicError, direction, fft, fft_plan_descriptor, fft_wrap_many,  # noqa: F401
-                   runtime_api, transpose_4d, fft_direction_name)
+                   runtime_api, transpose_4d, fft_direction_name, fill_random_real)
 from .ffts import (ffts, ffts_init_options, strategy_4d_pencil_pencil, strategy_4d_slab_slab,  # noqa: F401
                    strategy_4d_slab_native_xzwy, transpose_backend, transform_strategy_4d)
 

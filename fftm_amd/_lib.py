@@ -117,6 +117,7 @@ SIGNATURES = {
     "fftm_mi_ffts_load_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p, _i32p]),
     "fftm_mi_hardware_signature": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "fftm_mi_fnv1a64": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
+    "fftm_mi_fill_random_real": (ctypes.c_int, [_i32, _i32, _i64p, _i64p, _i64p, ctypes.c_uint64, _vp, _vp, _i32]),
     "fftm_mi_last_error": (ctypes.c_char_p, []),
     "fftm_mi_version": (ctypes.c_int, []),
 }

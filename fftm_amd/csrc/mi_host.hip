@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/fftm_mi.h"
+#include "mi_fill.hpp"
 #include "mi_launch.hpp"
 #include "mi_spectral.hpp"
 #include "mi_transpose.hpp"
@@ -1046,6 +1047,54 @@ const char *fftm_mi_last_error( void )
 int fftm_mi_version( void )
 {
     return 100; // 0.1.0
+}
+
+int fftm_mi_fill_random_real( int32_t precision, int32_t rank, const int64_t *n, const int64_t *strides,
+                              const int64_t *start, uint64_t seed, void *dst, void *stream, int32_t sync )
+{
+    if ( !n || !strides || !dst )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_fill_random_real: null argument" );
+    if ( rank != 3 && rank != 4 )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_fill_random_real: rank must be 3 or 4" );
+    if ( precision != FFTM_MI_F32 && precision != FFTM_MI_F64 )
+        return fail( FFTM_MI_INVALID_TYPE, "fftm_mi_fill_random_real: precision must be FFTM_MI_F32 or FFTM_MI_F64" );
+    fill_params p{};
+    p.nd    = rank;
+    p.seed  = seed;
+    p.total = 1;
+    int order[4];
+    for ( int a = 0; a < rank; ++a )
+    {
+        if ( n[a] < 1 || strides[a] < 1 )
+            return fail( FFTM_MI_INVALID_SIZE, "fftm_mi_fill_random_real: extents and strides must be >= 1" );
+        if ( start && start[a] < 0 )
+            return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_fill_random_real: start offsets must be >= 0" );
+        p.start[a] = start ? (uint64_t)start[a] : 0;
+        p.total *= n[a];
+        order[a] = a;
+    }
+    // memory order: largest stride first (ties keep the logical order)
+    std::stable_sort( order, order + rank, [&]( int a, int b ) { return strides[a] > strides[b]; } );
+    for ( int m = 0; m < rank; ++m )
+    {
+        p.ax[m]  = order[m];
+        p.ext[m] = n[order[m]];
+        p.str[m] = strides[order[m]];
+    }
+    hipStream_t   s      = static_cast<hipStream_t>( stream );
+    const int64_t blocks = std::min<int64_t>( ( p.total + 255 ) / 256, 16384 );
+    if ( precision == FFTM_MI_F32 )
+        hipLaunchKernelGGL( k_fill_random<float>, dim3( (unsigned)blocks ), dim3( 256 ), 0, s, p,
+                            static_cast<float *>( dst ) );
+    else
+        hipLaunchKernelGGL( k_fill_random<double>, dim3( (unsigned)blocks ), dim3( 256 ), 0, s, p,
+                            static_cast<double *>( dst ) );
+    hipError_t e = hipGetLastError();
+    if ( e == hipSuccess && sync )
+        e = hipStreamSynchronize( s );
+    if ( e != hipSuccess )
+        return fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_fill_random_real: " ) + hipGetErrorString( e ) );
+    return ok();
 }
 
 int fftm_mi_plan_validate( const fftm_mi_plan_desc *desc, size_t *work_size )

@@ -702,6 +702,35 @@ class fft_wrap_many:
             pass
 
 
+def fill_random_real(dst, logical_dims: Sequence[int], seed: int, logical_strides: Optional[Sequence[int]] = None,
+                     start: Optional[Sequence[int]] = None, stream=None, sync: bool = False):
+    """fill_random_real_3d_functor / fill_random_real_4d_functor of the benchmark harness
+    (source/tests/detail/fft_benchmark_common.h:201-257) run on the device into `dst`
+    (float32 or float64, 3 or 4 logical axes x, y, z[, w]).  `logical_strides` are the
+    element strides of the logical axes in `dst` (default: C order of `logical_dims`, i.e.
+    the 3D / slab real layouts); `start` is a slab's logical origin."""
+    import torch
+    nd = len(logical_dims)
+    if nd not in (3, 4):
+        raise ValueError("fill_random_real: 3 or 4 logical axes")
+    if dst.dtype not in (torch.float32, torch.float64):
+        raise TypeError("fill_random_real: dst must be float32 or float64")
+    if logical_strides is None:
+        logical_strides = [1] * nd
+        for a in range(nd - 2, -1, -1):
+            logical_strides[a] = logical_strides[a + 1] * int(logical_dims[a + 1])
+    span = 1 + sum((int(n) - 1) * int(s) for n, s in zip(logical_dims, logical_strides))
+    if span > dst.numel() or not dst.is_contiguous():
+        raise ValueError("fill_random_real: the strides address elements outside the contiguous tensor")
+    st = None if start is None else _lib.i64_array(start)
+    L = _lib.lib()
+    _lib.check(L.fftm_mi_fill_random_real(_lib.F32 if dst.dtype == torch.float32 else _lib.F64, nd,
+                                          _lib.i64_array(logical_dims), _lib.i64_array(logical_strides), st,
+                                          int(seed) & 0xFFFFFFFFFFFFFFFF, device_ptr(dst), stream_handle(stream),
+                                          1 if sync else 0), "fftm_mi_fill_random_real")
+    return dst
+
+
 def transpose_4d(src, dst, perm: Sequence[int], stream=None, sync: bool = False):
     """dst = transpose(src, perm) for 4D memory arrays (torch tensors, contiguous)."""
     if src.dim() != 4:

@@ -295,6 +295,19 @@ int fftm_mi_hardware_signature( char *buf, size_t len );
 /* FNV-1a 64 of a string as 16 lowercase hex digits (host only). */
 int fftm_mi_fnv1a64( const char *text, char *hex17 );
 
+/*
+ * The benchmark harness's deterministic input on the device: fill_random_real_3d_functor /
+ * fill_random_real_4d_functor with splitmix64 and unit_random_from_state<T>
+ * (source/tests/detail/fft_benchmark_common.h:201-257).  Logical element (x, y, z[, w])
+ * (rank 3 or 4; n[a] extents, strides[a] element strides of logical axis a in dst, any
+ * arrangement) gets 2 u - 1, u = (splitmix64(state) >> 11) 2^-53 rounded to T, with
+ * state = seed ^ gx 0xD6E8FEB86659FD93 ^ gy 0xA5A3564E27F886A5 ^ gz 0x9E3779B97F4A7C15
+ * [^ gw 0x94D049BB133111EB] and g = start[a] + index (start may be NULL: zeros; a slab's
+ * origin otherwise).  precision: FFTM_MI_F32 or FFTM_MI_F64.  Runs on `stream`.
+ */
+int fftm_mi_fill_random_real( int32_t precision, int32_t rank, const int64_t *n, const int64_t *strides,
+                              const int64_t *start, uint64_t seed, void *dst, void *stream, int32_t sync );
+
 /* Last error message of the calling thread ("" when none). */
 const char *fftm_mi_last_error( void );
 

@@ -250,3 +250,20 @@ def test_header_is_plain_c(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.split()[1] == str(512 * 512 * 257 * 8)
+
+
+def test_fill_random_real_rejects_bad_arguments_on_the_host():
+    """fftm_mi_fill_random_real validates before any launch (dst is never touched)."""
+    L = _lib()
+    lib = L.lib()
+    n = L.i64_array([4, 4, 4])
+    s = L.i64_array([16, 4, 1])
+    dummy = 0x1000
+    assert lib.fftm_mi_fill_random_real(7, 3, n, s, None, 1, dummy, None, 1) == L.INVALID_TYPE
+    assert lib.fftm_mi_fill_random_real(L.F32, 5, n, s, None, 1, dummy, None, 1) == L.INVALID_VALUE
+    assert lib.fftm_mi_fill_random_real(L.F32, 3, n, s, L.i64_array([-1, 0, 0]), 1, dummy, None, 1) \
+        == L.INVALID_VALUE
+    assert lib.fftm_mi_fill_random_real(L.F32, 3, L.i64_array([4, 0, 4]), s, None, 1, dummy, None, 1) \
+        == L.INVALID_SIZE
+    assert lib.fftm_mi_fill_random_real(L.F32, 3, n, s, None, 1, None, None, 1) == L.INVALID_VALUE
+    assert "fill_random_real" in L.last_error()

@@ -39,3 +39,19 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stderr
+
+
+def test_cpu_baseline_protocol_small():
+    """The CPU baseline leg on a tiny field: forward and backward timed apart, the
+    reference's statistics per pair, both precisions, and a round trip at rounding level."""
+    import numpy as np
+    b = _bench()
+    r = b.run_cpu_baseline((16, 12, 10), pairs=4, warmup=1)
+    assert r["kind"] == "port" and r["unit"] == "GFLOP/s" and r["cores"] >= 1
+    assert "pocketfft" in r["sample"] and "FFTW unavailable" in r["sample"]
+    for key, eps in (("f64", np.finfo(np.float64).eps), ("f32", np.finfo(np.float32).eps)):
+        leg = r[key]
+        assert leg["pair_ms"]["samples"] == 4 and leg["forward_ms"]["samples"] == 4
+        assert leg["pair_ms"]["min"] <= leg["pair_ms"]["median"] <= leg["pair_ms"]["max"]
+        assert leg["roundtrip_rel_l2"] < 10 * eps
+    assert r["value"] == r["f64"]["gflops"] > 0
