@@ -111,6 +111,7 @@ SIGNATURES = {
     "fftm_mi_ffts_c2r_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
     "fftm_mi_ffts_spectral_multiply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32]),
     "fftm_mi_ffts_shell_gradient": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32]),
+    "fftm_mi_ffts_shell_gradient_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32]),
     "fftm_mi_ffts_spectral_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32]),
     "fftm_mi_ffts_filter": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32]),
     "fftm_mi_ffts_autotune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, ctypes.c_char_p]),

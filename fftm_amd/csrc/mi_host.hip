@@ -13,6 +13,7 @@
 #include <map>
 #include <sstream>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -994,6 +995,8 @@ struct fftm_mi_ffts_s
     hipStream_t          stream    = nullptr;
     spec_geom            sg{};
     double              *partials  = nullptr; // shell-gradient partial sums (SHELL_BLOCKS x MAX_SHELLS)
+    double              *scratch   = nullptr; // this device's `scratch` (shell_gradient_noalloc)
+    hipError_t           scratch_err = hipSuccess;
     // filter chain real -> real: forward passes into the work area, the fused filter pass
     // (or the forward passes, an elementwise multiply and the backward passes)
     std::vector<pass_rt> flt_pre, flt_post;
@@ -1012,6 +1015,43 @@ namespace
 {
 constexpr int SHELL_BLOCKS = 512;
 constexpr int MAX_SHELLS   = 1024;
+} // namespace
+
+// Scratch of fftm_mi_ffts_shell_gradient_noalloc: the shell partial sums (SHELL_BLOCKS x
+// nshells doubles, 4 MiB at most), one copy per GPU for the life of the process.  Host
+// code never uses the array's name as an address: it uses the per-device pointer that
+// hipGetSymbolAddress returned for the device that was current at handle creation.
+static __device__ __align__( 256 ) double scratch[SHELL_BLOCKS * MAX_SHELLS];
+
+namespace
+{
+constexpr int MAX_DEVICES = 64;
+std::mutex    g_scratch_mu;
+void         *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's address of `scratch` (looked up once per device), or nullptr
+// with the HIP error in *err.
+void *scratch_for_current_device( hipError_t *err )
+{
+    *err    = hipSuccess;
+    int dev = 0;
+    if ( ( *err = hipGetDevice( &dev ) ) != hipSuccess )
+        return nullptr;
+    if ( dev < 0 || dev >= MAX_DEVICES )
+    {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk( g_scratch_mu );
+    if ( !g_scratch_ptr[dev] )
+    {
+        void *ptr = nullptr;
+        if ( ( *err = hipGetSymbolAddress( &ptr, HIP_SYMBOL( scratch ) ) ) != hipSuccess )
+            return nullptr;
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
 
 spec_geom make_spec_geom( const ffts_layout &L )
 {
@@ -1034,6 +1074,31 @@ spec_geom make_spec_geom( const ffts_layout &L )
     g.kmax2   = kmax2 > 0 ? kmax2 : 1; // all-ones sizes: every bin has k = 0
     g.nshells = 1;
     return g;
+}
+
+// grad[s] from the partial sums in `partials` (SHELL_BLOCKS x nshells doubles)
+void launch_shell_gradient( fftm_mi_ffts_s *h, const void *grad_spectrum, const void *spectrum, void *grad,
+                            int nshells, double *partials )
+{
+    spec_geom g = h->sg;
+    g.nshells   = nshells;
+    const size_t lds = sizeof( double ) * (size_t)nshells;
+    if ( h->prec == FFTM_MI_F32 )
+    {
+        hipLaunchKernelGGL( k_shell_partials<float>, dim3( SHELL_BLOCKS ), dim3( 256 ), lds, h->stream, g,
+                            static_cast<const cpx<float> *>( grad_spectrum ), static_cast<const cpx<float> *>( spectrum ),
+                            partials );
+        hipLaunchKernelGGL( k_shell_reduce<float>, dim3( ( nshells + 255 ) / 256 ), dim3( 256 ), 0, h->stream,
+                            partials, SHELL_BLOCKS, nshells, static_cast<float *>( grad ) );
+    }
+    else
+    {
+        hipLaunchKernelGGL( k_shell_partials<double>, dim3( SHELL_BLOCKS ), dim3( 256 ), lds, h->stream, g,
+                            static_cast<const cpx<double> *>( grad_spectrum ),
+                            static_cast<const cpx<double> *>( spectrum ), partials );
+        hipLaunchKernelGGL( k_shell_reduce<double>, dim3( ( nshells + 255 ) / 256 ), dim3( 256 ), 0, h->stream,
+                            partials, SHELL_BLOCKS, nshells, static_cast<double *>( grad ) );
+    }
 }
 } // namespace
 
@@ -1379,6 +1444,7 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
         return rc;
     p->work_size = ws;
     p->sg        = make_spec_geom( L );
+    p->scratch   = static_cast<double *>( scratch_for_current_device( &p->scratch_err ) );
     *h           = p.release();
     if ( const char *cache = std::getenv( "FFTM_MI_AUTOTUNE_CACHE" ) )
         if ( *cache )
@@ -1581,26 +1647,29 @@ int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, cons
             return fail( FFTM_MI_ALLOC_FAILED, std::string( "fftm_mi_ffts_shell_gradient: " ) + hipGetErrorString( e ) );
         }
     }
-    spec_geom g = h->sg;
-    g.nshells   = nshells;
-    const size_t lds = sizeof( double ) * (size_t)nshells;
-    if ( h->prec == FFTM_MI_F32 )
-    {
-        hipLaunchKernelGGL( k_shell_partials<float>, dim3( SHELL_BLOCKS ), dim3( 256 ), lds, h->stream, g,
-                            static_cast<const cpx<float> *>( grad_spectrum ), static_cast<const cpx<float> *>( spectrum ),
-                            h->partials );
-        hipLaunchKernelGGL( k_shell_reduce<float>, dim3( ( nshells + 255 ) / 256 ), dim3( 256 ), 0, h->stream,
-                            h->partials, SHELL_BLOCKS, nshells, static_cast<float *>( grad ) );
-    }
-    else
-    {
-        hipLaunchKernelGGL( k_shell_partials<double>, dim3( SHELL_BLOCKS ), dim3( 256 ), lds, h->stream, g,
-                            static_cast<const cpx<double> *>( grad_spectrum ),
-                            static_cast<const cpx<double> *>( spectrum ), h->partials );
-        hipLaunchKernelGGL( k_shell_reduce<double>, dim3( ( nshells + 255 ) / 256 ), dim3( 256 ), 0, h->stream,
-                            h->partials, SHELL_BLOCKS, nshells, static_cast<double *>( grad ) );
-    }
+    launch_shell_gradient( h, grad_spectrum, spectrum, grad, nshells, h->partials );
     return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_shell_gradient" );
+}
+
+int fftm_mi_ffts_shell_gradient_noalloc( fftm_mi_ffts h, const void *grad_spectrum, const void *spectrum, void *grad,
+                                         int32_t nshells, int32_t sync )
+{
+    if ( !h )
+        return fail( FFTM_MI_INVALID_PLAN, "fftm_mi_ffts_shell_gradient_noalloc: null handle" );
+    if ( !grad_spectrum || !spectrum || !grad )
+        return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_shell_gradient_noalloc: null data pointer" );
+    if ( nshells < 1 || nshells > MAX_SHELLS )
+        return fail( FFTM_MI_INVALID_VALUE,
+                     "fftm_mi_ffts_shell_gradient_noalloc: nshells must be in 1..1024 (the static scratch holds "
+                     "512 x 1024 partial sums)" );
+    if ( !h->scratch )
+        return fail( FFTM_MI_SETUP_FAILED, std::string( "fftm_mi_ffts_shell_gradient_noalloc: no device address for "
+                                                        "the static scratch: " ) +
+                                               hipGetErrorString( h->scratch_err ) );
+    // every partial sum the reduction reads is written by the partials kernel of this
+    // call (SHELL_BLOCKS x nshells), so the scratch needs no clearing
+    launch_shell_gradient( h, grad_spectrum, spectrum, grad, nshells, h->scratch );
+    return spectral_launch_check( h, sync != 0, "fftm_mi_ffts_shell_gradient_noalloc" );
 }
 
 } // extern "C"

@@ -276,6 +276,19 @@ int fftm_mi_ffts_shell_gradient( fftm_mi_ffts h, const void *grad_spectrum, cons
                                  int32_t nshells, int32_t sync );
 
 /*
+ * fftm_mi_ffts_shell_gradient without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The shell partial sums live in the library's
+ * static device scratch (512 x 1024 doubles, 4 MiB, one copy per GPU held until the process
+ * ends) instead of a buffer allocated on the first call; nshells is limited to 1..1024 as
+ * for fftm_mi_ffts_shell_gradient.  Every call on a GPU shares that GPU's scratch: calls
+ * that overlap in time on one GPU (other streams, other threads) race and must be ordered
+ * by the caller.  Same arguments and results as fftm_mi_ffts_shell_gradient.
+ */
+int fftm_mi_ffts_shell_gradient_noalloc( fftm_mi_ffts h, const void *grad_spectrum, const void *spectrum, void *grad,
+                                         int32_t nshells, int32_t sync );
+
+/*
  * Tuning of the tile-shape variants with the reference's autotune cache protocol
  * (source/fftm_autotune.hpp:126-169, fftm_autotune_measure.hpp:293-400,
  * detail/fftm_autotune_hardware.inc:87-98, 285-337): for every pass of the handle and
