@@ -127,7 +127,7 @@ def tune_handle(f, warmup, iterations):
     return f.autotune(warmup=warmup, iterations=iterations, cache_path=TUNING_CACHE)
 
 
-def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None):
+def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None, destructive=False):
     """Time the forward + backward pair.  check: True (FP64 pocketfft on the host, same
     input bytes), "gpu64" (this library's FP64 transform of the same field, for sizes where
     the host transform takes too long) or False.  u_dev: a device input (else the
@@ -140,7 +140,7 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
     dt = np.float32
     f = fftm_amd.ffts("float32", strategy_4d=strategy)
     opts = fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key="", execution="fused", sync=False,
-                                      spectrum_storage=storage)
+                                      spectrum_storage=storage, destructive_backward=destructive)
     f.init(*dims, options=opts)
     stream = torch.cuda.current_stream()
     f.set_stream(stream)
@@ -710,6 +710,17 @@ def main():
                                   "ms_per_pair": round(r256["api_pair_s"] * 1e3, 4),
                                   "ms_per_pair_timed_region": round(r256["pair_s"] * 1e3, 4),
                                   "hbm_gbps_alg": round(_b_alg((256,) * 3, 4) / r256["api_pair_s"] / 1e9, 1)}
+        # the backward allowed to overwrite its input (hipfftExecC2R's contract, which the
+        # reference's 3D backward inherits, ffts.hpp:363-368): every backward pass in place
+        rd = run_gpu(dims, fftm_amd.strategy_4d_pencil_pencil(), args.steps, args.warmup, False, not args.no_tune,
+                     destructive=True)
+        out["destructive_backward_512^3"] = {
+            "workload": f"ffts 3D pair {n}^3 FP32, backward overwrites its input spectrum (no work area)",
+            "ms_per_pair": round(rd["pair_s"] * 1e3, 4),
+            "gflops": round(_flops_pair(N) / rd["pair_s"] / 1e9, 2),
+            "hbm_gbps_alg": round(balg / rd["pair_s"] / 1e9, 1),
+            "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in rd["per_pass_ms"].items()},
+            "pass_variants": rd["variants"]}
         # FP16-stored spectra against the plain FP32 pair: same field, same harness
         rm = run_gpu(dims, None, args.steps, args.warmup, not args.no_check, not args.no_tune, storage="fp16")
         mp = {"workload": f"ffts 3D pair {n}^3, FP32 arithmetic, FP16 half spectra between passes "

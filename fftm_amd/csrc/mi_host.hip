@@ -342,9 +342,10 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.wmode   = r.wmode;
     p.vec     = pair_access_bits( r, src, dst );
     p.scale   = (T)r.scale;
-    // tile shape (compile-time kernel) and tile order (runtime bit)
+    // tile shape (compile-time kernel), tile order (runtime bit) and radix schedule
     const int  shape = variant_shape_of( r.var );
     const int  rr    = variant_vec_order( r.var );
+    const int  rs    = variant_rs_of( r.var );
     p.vec |= rr;
     if constexpr ( std::is_same<T, float>::value )
     {
@@ -360,8 +361,12 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
             p.ys0            = r.ys0;
             p.ys1            = r.ys1;
             p.ys2            = r.ys2;
+            if ( rs )
+                return launch_pass_rs_f32( p, (int)r.N, r.inv, shape, rs, r.iom, s );
             return launch_pass_mixed_f32( p, (int)r.N, r.kind, r.inv, r.loadb, shape, r.iom, s );
         }
+        if ( rs )
+            return launch_pass_rs_f32( p, (int)r.N, r.inv, shape, rs, IO_F32, s );
         if ( r.filter )
             return launch_filter_f32( p, *static_cast<const filter_params<float> *>( filter ), (int)r.N, r.loadb, shape, s );
         return r.generic ? launch_generic_f32( p, r.rl, (int)r.N, r.gen_cols, r.kind, r.inv, s )
@@ -414,6 +419,20 @@ int default_order( const pass_rt &p )
 {
     const bool big = p.plane ? p.nplanes >= 2048 : p.ncols >= 32768;
     return big ? 3 : 0;
+}
+
+// a variant number this pass can run: FP16-storage passes have tile shapes 0, 6, 7; the
+// alternative radix schedules exist for the strided FP32-arithmetic C2C passes of lengths
+// 512 and 1024 in tile shapes 0 and 6
+bool pass_variant_ok( const pass_rt &r, int var, int prec )
+{
+    if ( !variant_number_ok( var ) )
+        return false;
+    if ( r.iom && !mixed_variant_ok( var ) )
+        return false;
+    const int rs = variant_rs_of( var );
+    return rs == 0 ||
+           ( !r.plane && !r.generic && !r.filter && rs_supported( r.N, r.kind, r.loadb, prec, variant_shape_of( var ) ) );
 }
 
 int prepare_passes( std::vector<pass_rt> &passes, int prec, tables *tb )
@@ -1764,7 +1783,7 @@ int ensure_filter_chain( fftm_mi_ffts_s &h )
         const int lv      = variant_shape_of( lf.var );
         pre.back().var    = h.flt_var >= 0 ? h.flt_var
                                            : ( ( lv == 2 || lv == 4 ) ? lv : ( ( lv == 3 || lv == 5 ) ? 4 : 0 ) ) |
-                                                 ( lf.var & ~7 );
+                                                 ( lf.var & ( 3 << 3 ) );
         post.erase( post.begin() );
     }
     h.flt_pre     = pre;
@@ -1852,6 +1871,10 @@ int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t ind
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: variant out of range" );
     if ( v[(size_t)index].iom && !mixed_variant_ok( variant ) )
         return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_set_pass_variant: FP16-storage passes have tile shapes 0, 6, 7" );
+    if ( !pass_variant_ok( v[(size_t)index], variant, h->prec ) )
+        return fail( FFTM_MI_NOT_SUPPORTED,
+                     "fftm_mi_ffts_set_pass_variant: radix schedules 1 and 2 exist for strided C2C passes of length 512 "
+                     "and 1024 in tile shapes 0 and 6 (FP32 arithmetic)" );
     v[(size_t)index].var = variant;
     h->flt_ready         = false;
     return ok();
@@ -2048,6 +2071,63 @@ hipError_t tune_compare( const fftm_mi_ffts_s &h, void *ref, const void *buf, si
     return he;
 }
 
+// mx[0] = max |a - b| (a NaN counts as infinite), mx[1] = max |b|, as the bits of
+// non-negative floats (which order like the unsigned integers)
+__global__ void __launch_bounds__( 256 ) k_absdiff_max( const float *a, const float *b, int64_t n, unsigned *mx )
+{
+    __shared__ float sd[256], sm[256];
+    float            d = 0.f, m = 0.f;
+    for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256 )
+    {
+        const float x = a[i], y = b[i];
+        const float e = fabsf( x - y );
+        d             = fmaxf( d, e == e ? e : __builtin_inff() );
+        m             = fmaxf( m, fabsf( y ) );
+    }
+    sd[threadIdx.x] = d;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for ( int w = 128; w > 0; w >>= 1 )
+    {
+        if ( (int)threadIdx.x < w )
+        {
+            sd[threadIdx.x] = fmaxf( sd[threadIdx.x], sd[threadIdx.x + w] );
+            sm[threadIdx.x] = fmaxf( sm[threadIdx.x], sm[threadIdx.x + w] );
+        }
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+    {
+        atomicMax( mx, __float_as_uint( sd[0] ) );
+        atomicMax( mx + 1, __float_as_uint( sm[0] ) );
+    }
+}
+
+// buf (n bytes of FP32 values) against the captured ref: max |buf - ref| <= tol max |ref|
+hipError_t tune_compare_tol( const fftm_mi_ffts_s &h, const void *ref, const void *buf, size_t n, double tol,
+                             unsigned long long *scratch8, bool *ok )
+{
+    *ok              = false;
+    unsigned    *mx  = reinterpret_cast<unsigned *>( scratch8 );
+    unsigned     hm[2] = { 0, 0 };
+    hipError_t   he  = hipMemsetAsync( mx, 0, 2 * sizeof( unsigned ), h.stream );
+    if ( he == hipSuccess )
+    {
+        hipLaunchKernelGGL( k_absdiff_max, dim3( 1024 ), dim3( 256 ), 0, h.stream, static_cast<const float *>( buf ),
+                            static_cast<const float *>( ref ), (int64_t)( n / 4 ), mx );
+        he = hipGetLastError();
+    }
+    if ( he == hipSuccess )
+        he = hipMemcpyAsync( hm, mx, sizeof( hm ), hipMemcpyDeviceToHost, h.stream );
+    if ( he == hipSuccess )
+        he = hipStreamSynchronize( h.stream );
+    float d = 0.f, m = 0.f;
+    std::memcpy( &d, &hm[0], 4 );
+    std::memcpy( &m, &hm[1], 4 );
+    *ok = std::isfinite( d ) && d <= tol * m;
+    return he;
+}
+
 // The fused filter pass (forward passes, x H(k) inside the last one, inverse passes) in
 // each of its variants, with a Gaussian low-pass as the operator: the fastest one whose
 // output is bitwise equal to variant 0's is kept in h.flt_var.  Layouts whose filter
@@ -2184,7 +2264,7 @@ int fftm_mi_ffts_load_tuning( fftm_mi_ffts h, const char *cache_path, int32_t *a
             if ( it == m.end() )
                 continue;
             const int var = std::atoi( it->second.c_str() );
-            if ( variant_number_ok( var ) && ( !v[i].iom || mixed_variant_ok( var ) ) )
+            if ( pass_variant_ok( v[i], var, h->prec ) )
             {
                 v[i].var = var;
                 ++n;
@@ -2268,10 +2348,15 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
             // passes have one shape and their own order)
             const int            nvar    = cur.generic ? 1 : ( cur.plane ? 2 : fixed_variants( cur.N ) );
             const int            ncand   = cur.generic ? 1 : NUM_ORDERS * nvar;
-            for ( int cand = 0; cand < ncand && rc == 0; ++cand )
+            // radix schedules 1 and 2 (alternative schedule, and its radix-16 stage on the
+            // matrix cores) where compiled, for passes with FP32 output (their validity is a
+            // tolerance against variant 0's output, see below)
+            const int            nrs     = cur.iom ? 1 : 3;
+            for ( int cand = 0; cand < ncand * nrs && rc == 0; ++cand )
             {
-                const int var = cand % nvar + 8 * ( cand / nvar );
-                if ( cur.iom && !mixed_variant_ok( var ) )
+                const int base = cand % ncand;
+                const int var  = base % nvar + 8 * ( base / nvar ) + 32 * ( cand / ncand );
+                if ( !pass_variant_ok( cur, var, h->prec ) )
                     continue;
                 one[0].var = var;
                 // regenerate the pass input, run once, check against variant 0
@@ -2285,8 +2370,15 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
                     break;
                 // every copy, clear and comparison is ordered on the handle's stream (which
                 // may be non-blocking with respect to the null stream)
+                // valid: bitwise equal to variant 0 (same radix schedule: same arithmetic in
+                // another tile shape or order), or, for another radix schedule, within the
+                // FP32 parity tolerance of variant 0's output (relative max 1e-5, tighter than
+                // the reference's 1000 eps log2 N round-trip check,
+                // fftm_autotune_measure.hpp:837-845)
                 bool       valid = true;
-                hipError_t he    = tune_compare( *h, ref, dstbuf, dbytes, var == 0, dcount, &valid );
+                hipError_t he    = variant_rs_of( var ) == 0
+                                       ? tune_compare( *h, ref, dstbuf, dbytes, var == 0, dcount, &valid )
+                                       : tune_compare_tol( *h, ref, dstbuf, dbytes, 1e-5, dcount, &valid );
                 if ( he != hipSuccess )
                 {
                     rc = fail( FFTM_MI_EXEC_FAILED, std::string( "fftm_mi_ffts_autotune: " ) + hipGetErrorString( he ) );

@@ -130,6 +130,26 @@ hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, 
     }
 }
 
+hipError_t launch_rs_f32_n512( const pass_params<float> &, bool, int, int, int, hipStream_t );
+hipError_t launch_rs_f32_n1024( const pass_params<float> &, bool, int, int, int, hipStream_t );
+
+bool rs_supported( int64_t N, int kind, bool loadb, int prec, int shape )
+{
+    return ( N == 512 || N == 1024 ) && kind == KIND_C2C && loadb && prec == 0 && ( shape == 0 || shape == 6 );
+}
+
+hipError_t launch_pass_rs_f32( const pass_params<float> &p, int N, bool inv, int shape, int rs, int iom, hipStream_t s )
+{
+    if ( rs < 1 || rs > 2 )
+        return hipErrorInvalidValue;
+    switch ( N )
+    {
+    case 512: return launch_rs_f32_n512( p, inv, shape, rs, iom, s );
+    case 1024: return launch_rs_f32_n1024( p, inv, shape, rs, iom, s );
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_generic_f32( const pass_params<float> &p, const radix_list &rl, int N, int cols, int kind, bool inv,
                                hipStream_t s )
 {

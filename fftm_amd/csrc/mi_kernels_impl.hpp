@@ -16,17 +16,18 @@ constexpr int variants_for()
     return ( N & ( N - 1 ) ) == 0 && N >= 64 ? NUM_VARIANTS : 1;
 }
 
-template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32>
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32, int RS = 0>
 hipError_t launch_fixed( const pass_params<T> &p, hipStream_t s )
 {
-    using K              = pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>;
+    using K              = pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM, false, RS>;
     const int64_t per    = (int64_t)K::TW * K::CPT;
     const int64_t ntiles = ( p.ncols + per - 1 ) / per;
     if ( ntiles <= 0 )
         return hipSuccess;
     if ( ntiles > 0x7fffffffLL )
         return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL( ( k_pass<T, N, KIND, INV, LOADB, VAR, IOM> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s, p );
+    hipLaunchKernelGGL( ( k_pass<T, N, KIND, INV, LOADB, VAR, IOM, RS> ), dim3( (unsigned)ntiles ), dim3( K::NT ), 0, s,
+                        p );
     return hipGetLastError();
 }
 
@@ -201,6 +202,40 @@ hipError_t launch_generic_t( const pass_params<T> &p, const radix_list &rl, int 
     else
         hipLaunchKernelGGL( ( k_pass_generic<T, KIND_C2R, true> ), grid, block, lds, s, p, rl, N, cols );
     return hipGetLastError();
+}
+
+// Alternative radix schedules (RS 1: make_rplan_alt on the vector ALUs, RS 2: its radix-16
+// stage on the matrix cores) of the strided C2C passes, FP32 arithmetic with FP32 or
+// FP16-stored lines, tile shapes 0 and 6.
+template <int N, int VAR, int RS>
+hipError_t launch_rs_v( const pass_params<float> &p, bool inv, int iom, hipStream_t s )
+{
+    switch ( iom )
+    {
+    case IO_F32:
+        return inv ? launch_fixed<float, N, KIND_C2C, true, true, VAR, IO_F32, RS>( p, s )
+                   : launch_fixed<float, N, KIND_C2C, false, true, VAR, IO_F32, RS>( p, s );
+    case IO_H16_IN:
+        return inv ? launch_fixed<float, N, KIND_C2C, true, true, VAR, IO_H16_IN, RS>( p, s )
+                   : launch_fixed<float, N, KIND_C2C, false, true, VAR, IO_H16_IN, RS>( p, s );
+    case IO_H16_OUT:
+        return inv ? launch_fixed<float, N, KIND_C2C, true, true, VAR, IO_H16_OUT, RS>( p, s )
+                   : launch_fixed<float, N, KIND_C2C, false, true, VAR, IO_H16_OUT, RS>( p, s );
+    case IO_H16_IN | IO_H16_OUT:
+        return inv ? launch_fixed<float, N, KIND_C2C, true, true, VAR, IO_H16_IN | IO_H16_OUT, RS>( p, s )
+                   : launch_fixed<float, N, KIND_C2C, false, true, VAR, IO_H16_IN | IO_H16_OUT, RS>( p, s );
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int N>
+hipError_t launch_rs_n( const pass_params<float> &p, bool inv, int shape, int rs, int iom, hipStream_t s )
+{
+    if ( shape == 0 )
+        return rs == 1 ? launch_rs_v<N, 0, 1>( p, inv, iom, s ) : launch_rs_v<N, 0, 2>( p, inv, iom, s );
+    if ( shape == 6 )
+        return rs == 1 ? launch_rs_v<N, 6, 1>( p, inv, iom, s ) : launch_rs_v<N, 6, 2>( p, inv, iom, s );
+    return hipErrorInvalidValue;
 }
 
 } // namespace detail

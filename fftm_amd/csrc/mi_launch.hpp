@@ -59,11 +59,17 @@ inline bool mixed_variant_ok( int var )
     const int v = variant_shape_of( var );
     return v == 0 || v == 6 || v == 7;
 }
-// a variant number: a tile shape 0..NUM_VARIANTS-1 plus 8 * the tile order
+// a variant number: a tile shape 0..NUM_VARIANTS-1, plus 8 * the tile order (0..3), plus 32 *
+// the radix schedule (0 default; 1 the alternative schedule; 2 the alternative schedule with
+// its radix-16 stage on the matrix cores; 1 and 2 exist where rs_supported)
 inline bool variant_number_ok( int var )
 {
-    return var >= 0 && variant_shape_of( var ) < NUM_VARIANTS && variant_order_of( var ) < NUM_ORDERS;
+    return var >= 0 && variant_rs_of( var ) <= 2;
 }
+// alternative radix schedules are compiled for strided C2C passes of length 512 and 1024
+// with FP32 arithmetic (prec 0), tile shapes 0 and 6
+bool       rs_supported( int64_t N, int kind, bool loadb, int prec, int shape );
+hipError_t launch_pass_rs_f32( const pass_params<float> &p, int N, bool inv, int shape, int rs, int iom, hipStream_t s );
 // the order bits of pass_params::vec for a variant number
 inline int variant_vec_order( int var )
 {

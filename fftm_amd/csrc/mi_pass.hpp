@@ -29,6 +29,7 @@
 #pragma once
 
 #include "mi_complex.hpp"
+#include "mi_mfma.hpp"
 
 #include <type_traits>
 
@@ -269,6 +270,31 @@ constexpr rplan make_rplan( int N, bool big_last = false )
     return p;
 }
 
+// The alternative radix schedule of a power-of-two length (a tuning candidate): radix 8,
+// then radix 16 in the second stage, then radix 8s and a final 4 or 2, e.g. 512 = 8 16 4,
+// 1024 = 8 16 8.  Its radix-16 stage is an LDS-to-LDS stage, which the matrix-core variant
+// runs as FP16 MFMA products (dft16_mfma).
+constexpr rplan make_rplan_alt( int N )
+{
+    int k = 0;
+    for ( int m = N; m > 1 && m % 2 == 0; m /= 2 )
+        ++k;
+    if ( ( 1 << k ) != N || k < 7 )
+        return make_rplan( N );
+    rplan p{ 0, {} };
+    p.r[p.n++] = 8;
+    p.r[p.n++] = 16;
+    k -= 7;
+    while ( k >= 3 )
+    {
+        p.r[p.n++] = 8;
+        k -= 3;
+    }
+    if ( k > 0 )
+        p.r[p.n++] = 1 << k;
+    return p;
+}
+
 constexpr int rplan_max( const rplan &p )
 {
     int r = 1;
@@ -325,7 +351,8 @@ constexpr int VEC_RR      = 4;
 constexpr int VEC_CHUNK8  = 8;
 constexpr int VEC_CHUNK32 = 16;
 constexpr int variant_shape_of( int var ) { return var & 7; }
-constexpr int variant_order_of( int var ) { return var >> 3; }
+constexpr int variant_order_of( int var ) { return ( var >> 3 ) & 3; }
+constexpr int variant_rs_of( int var ) { return var >> 5; }
 
 __device__ __forceinline__ int64_t ordered_tile( int order_bits )
 {
@@ -564,9 +591,15 @@ __device__ __forceinline__ void store_line( const pass_params<T> &p, int64_t oa,
 // ---------------------------------------------------------------------------
 // the compile-time-N pass kernel
 // ---------------------------------------------------------------------------
-template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR = 0, int IOM = IO_F32, bool FLT = false>
+// RS selects the radix schedule: 0 the default (make_rplan), 1 the alternative
+// (make_rplan_alt), 2 the alternative with its radix-16 stage on the matrix cores
+// (FP16 hi + lo operands, FP32 accumulation; strided FP32-arithmetic C2C passes)
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR = 0, int IOM = IO_F32, bool FLT = false,
+          int RS = 0>
 struct pass_kernel
 {
+    static_assert( RS == 0 || ( KIND == KIND_C2C && LOADB && !FLT && std::is_same<T, float>::value ),
+                   "alternative radix schedules are compiled for strided FP32 C2C passes" );
     static_assert( !FLT || ( KIND == KIND_C2C && !INV && IOM == IO_F32 && !variant_split( VAR ) ),
                    "the fused filter pass is a forward C2C pass with whole-value LDS exchange" );
     static_assert( IOM == IO_F32 || std::is_same<T, float>::value, "FP16 storage is for FP32 transforms" );
@@ -574,7 +607,7 @@ struct pass_kernel
     static_assert( !( IOM & IO_H16_OUT ) || KIND != KIND_C2R, "C2R writes real FP32 output" );
     using C                    = cpx<T>;
     // R2C restages its output through LDS, so only its stage-0 loads want pairs
-    static constexpr rplan RP  = make_rplan( N, KIND == KIND_R2C );
+    static constexpr rplan RP  = RS ? make_rplan_alt( N ) : make_rplan( N, KIND == KIND_R2C );
     static constexpr bool  SPLIT = variant_split( VAR );
     // FP16-stored lines move 4 bytes per bin: their variant 7 doubles the columns (128-byte
     // FP16 row segments) with the split exchange, instead of capping registers
@@ -1046,6 +1079,12 @@ struct pass_kernel
                 }
             }
             C w[NB2][R2];
+            // the matrix-core radix-16 stage: every lane of the wave holds whole butterflies
+            constexpr bool MF = RS == 2 && R2 == 16;
+            static_assert( !MF || NBF2 % TPC == 0, "matrix-core stage needs whole butterflies on every lane" );
+            mf_h8 am[4];
+            if constexpr ( MF )
+                dft16_mfma_operands<INV>( am, reinterpret_cast<const cpx<float> *>( x.p->tw ), N );
             if constexpr ( SPLIT )
             {
                 put_stage<0, S, R, NS, NB>( v, x );
@@ -1078,7 +1117,10 @@ struct pass_kernel
 #pragma unroll
                     for ( int r = 1; r < R2; ++r )
                         w[b][r] = twmul<INV>( w[b][r], tws[b][r] );
-                    dft_fixed<R2, INV>( w[b], x.p->tw, N );
+                    if constexpr ( MF )
+                        dft16_mfma( reinterpret_cast<cpx<float>( & )[16]>( w[b] ), am );
+                    else
+                        dft_fixed<R2, INV>( w[b], x.p->tw, N );
                 }
             }
             __syncthreads();
@@ -1356,12 +1398,12 @@ struct pass_kernel
     }
 };
 
-template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32>
-__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::NT ) )
-    __attribute__( ( amdgpu_waves_per_eu( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::WAVES_PER_EU ) ) )
+template <typename T, int N, int KIND, bool INV, bool LOADB, int VAR, int IOM = IO_F32, int RS = 0>
+__global__ void __launch_bounds__( ( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM, false, RS>::NT ) )
+    __attribute__( ( amdgpu_waves_per_eu( pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM, false, RS>::WAVES_PER_EU ) ) )
     k_pass( const pass_params<T> p )
 {
-    pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM>::run( p );
+    pass_kernel<T, N, KIND, INV, LOADB, VAR, IOM, false, RS>::run( p );
 }
 
 // fused filter pass: forward C2C along the lines, x H(k), inverse C2C, one HBM round trip

@@ -7,7 +7,8 @@ through properties that need no CPU transform of the whole field:
 * the round trip C2R(R2C(u)) = N u, to the reference's benchmark epsilon 10 eps(T)
   (fft_benchmark_common.h:28-32);
 * Parseval with Hermitian weights on the halved axis, and linearity;
-* the tuned variants giving bitwise the same spectrum as the defaults."""
+* the tuned variants giving bitwise the same spectrum as the defaults (within the FP32
+  tolerance where the tuner picks another radix schedule)."""
 
 import math
 
@@ -84,11 +85,16 @@ def test_512_cubed_fp32_properties(gpu):
     assert lin <= 1e-5
     del b, B, C, y
 
-    # tuned variants: bitwise the same spectrum
-    f.autotune(warmup=1, iterations=3)
+    # tuned variants: bitwise the same spectrum in the default radix schedules (variant
+    # numbers below 32); the alternative schedules within the FP32 parity tolerance
+    chosen = f.autotune(warmup=1, iterations=3)
     A2 = f.complex_array()
     f.forward(a, A2)
-    assert torch.equal(A2, A)
+    print(f"tuned variants {chosen}", flush=True)
+    if all(v < 32 for v in chosen.values()):
+        assert torch.equal(A2, A)
+    else:
+        assert float((A2 - A).abs().max() / A.abs().max()) <= 1e-5
     f.close()
 
 
