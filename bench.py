@@ -127,7 +127,8 @@ def tune_handle(f, warmup, iterations):
     return f.autotune(warmup=warmup, iterations=iterations, cache_path=TUNING_CACHE)
 
 
-def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None, destructive=False):
+def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_dev=None, destructive=False,
+            radix_schedule=None):
     """Time the forward + backward pair.  check: True (FP64 pocketfft on the host, same
     input bytes), "gpu64" (this library's FP64 transform of the same field, for sizes where
     the host transform takes too long) or False.  u_dev: a device input (else the
@@ -150,6 +151,15 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
     # every variant of every pass timed, fastest bitwise-valid one kept); with
     # --tuning-cache, a cache that already holds this problem is applied instead
     variants = tune_handle(f, warmup=2, iterations=7) if tune else f.pass_variants()
+    if radix_schedule is not None:
+        # the strided C2C passes (the y and x passes of a 3D transform) in the given radix
+        # schedule, keeping their tuned tile order (tile shape 0 or 6)
+        nf_, nb_ = f.num_passes()
+        for key in ((0, nf_ - 2), (0, nf_ - 1), (1, 0), (1, 1)):
+            v = variants[key]
+            shape = v & 7 if (v & 7) in (0, 6) else 6
+            f.set_pass_variant(key[0], key[1], shape + (v & 24) + 32 * radix_schedule)
+        variants = f.pass_variants()
 
     # input: splitmix64 uniform[-1,1) (fill_random_real_{3d,4d}_functor) generated on the
     # device by the library's fill kernel, resident in HBM
@@ -738,13 +748,38 @@ def main():
         if "fwd_rel_l2_vs_fp64" in rm:
             mp["meets_1e-5_rms"] = bool(rm["fwd_rel_l2_vs_fp64"] <= 1e-5)
         out["mixed_precision"] = mp
+
+        def leg(r, what):
+            d = {"workload": what, "ms_per_pair": round(r["pair_s"] * 1e3, 4),
+                 "speedup_vs_fp32": round(r3["pair_s"] / r["pair_s"], 3), "bytes_per_pair": r["bytes_per_pair"],
+                 "per_pass_ms": {f"{'bwd' if p[0] else 'fwd'}{p[1]}": round(v, 4) for p, v in r["per_pass_ms"].items()},
+                 "pass_variants": r["variants"]}
+            for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64", "roundtrip_rel_rms"):
+                if k in r:
+                    d[k] = r[k]
+            if "fwd_rel_l2_vs_fp64" in r:
+                d["meets_1e-5_rms"] = bool(r["fwd_rel_l2_vs_fp64"] <= 1e-5)
+            return d
+        # hi + lo FP16 storage: FP32-level accuracy at FP32's bytes
+        rx = run_gpu(dims, None, args.steps, args.warmup, not args.no_check, not args.no_tune, storage="fp16x2")
+        mp["hi_lo_fp16_storage"] = leg(rx, f"{n}^3 pair, half spectra stored as hi + lo FP16 pairs "
+                                           "(8 bytes per bin, per-line exponents)")
+        # the radix-16 stage of the strided passes as FP16 MFMA products (hi + lo operands,
+        # FP32 accumulation), with FP32 and with FP16-stored spectra
+        for storage in ("fp32", "fp16"):
+            rr = run_gpu(dims, None, args.steps, args.warmup, not args.no_check, not args.no_tune, storage=storage,
+                         radix_schedule=2)
+            mp[f"mfma_radix16_{storage}_storage"] = leg(
+                rr, f"{n}^3 pair, {storage} spectra, y and x passes in the 8 16 4 schedule with the radix-16 stage "
+                    "on the matrix cores (v_mfma_f32_32x32x16_f16, hi + lo operands, FP32 accumulation)")
+        _progress("hi + lo storage and matrix-core radix-16 legs done")
         _progress(f"256^3 and {n}^3 FP16-storage legs done")
         if args.big:
             # 1024^3: uniform(-1,1) + k^-5/3 field; FP64 reference from this library on the GPU
             import torch
             ub = kolmogorov_field(1024)
             big = {}
-            for storage in ("fp32", "fp16"):
+            for storage in ("fp32", "fp16", "fp16x2"):
                 rb = run_gpu((1024,) * 3, None, max(3, args.steps // 4), 2, "gpu64", not args.no_tune,
                              storage=storage, u_dev=ub)
                 big[storage] = {"ms_per_pair": round(rb["pair_s"] * 1e3, 4),
@@ -754,6 +789,7 @@ def main():
                                                       "roundtrip_rel_rms") if k in rb}}
             _progress("1024^3 FP32 and FP16-storage legs done")
             big["speedup_fp16_vs_fp32"] = round(big["fp32"]["ms_per_pair"] / big["fp16"]["ms_per_pair"], 3)
+            big["speedup_fp16x2_vs_fp32"] = round(big["fp32"]["ms_per_pair"] / big["fp16x2"]["ms_per_pair"], 3)
             big["input"] = "uniform(-1,1) + k^(-5/3) random-phase field (half the RMS), 1024^3 FP32"
             big["reference"] = "this library's FP64 transform of the same field on the GPU"
             out["mixed_precision"]["1024^3"] = big

@@ -21,6 +21,7 @@ SOURCES = ["mi_kernels_f32_a.hip", "mi_kernels_f32_b.hip", "mi_kernels_f32_c.hip
            "mi_kernels_f32_s0.hip", "mi_kernels_f32_s1.hip", "mi_kernels_f32_s2.hip", "mi_kernels_f32_s3.hip",
            "mi_kernels_f64_s0.hip", "mi_kernels_f64_s1.hip", "mi_kernels_f64_s2.hip", "mi_kernels_f64_s3.hip",
            "mi_kernels_mixed_a.hip", "mi_kernels_mixed_b.hip", "mi_kernels_rs_512.hip", "mi_kernels_rs_1024.hip",
+           "mi_kernels_mixed_s_a.hip", "mi_kernels_mixed_s_b.hip", "mi_kernels_mixed_s_c.hip",
            "mi_kernels_filter_f32.hip",
            "mi_kernels_filter_f64.hip",
            "mi_kernels_dispatch.hip", "mi_kernels_plane.hip", "mi_host.hip"]

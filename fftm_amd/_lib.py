@@ -30,6 +30,7 @@ NOT_SUPPORTED = 16
 R2C, C2R, C2CF, C2CB = 1, 2, 3, 4
 F32, F64 = 0, 1
 F32_H16 = 2  # FP32 transforms, FP16-stored intermediate spectra
+F32_H16X2 = 3  # ... stored as hi + lo FP16 pairs (8 bytes per bin)
 FLAG_OVERWRITE_INPUT = 1
 FLAG_NO_PLANE_FUSION = 2
 

@@ -431,6 +431,8 @@ bool pass_variant_ok( const pass_rt &r, int var, int prec )
     if ( r.iom && !mixed_variant_ok( var ) )
         return false;
     const int rs = variant_rs_of( var );
+    if ( ( r.iom & IO_SPLIT ) && ( variant_shape_of( var ) == 7 || rs != 0 ) )
+        return false; // hi + lo storage: tile shapes 0 and 6, default radix schedule
     return rs == 0 ||
            ( !r.plane && !r.generic && !r.filter && rs_supported( r.N, r.kind, r.loadb, prec, variant_shape_of( var ) ) );
 }
@@ -443,9 +445,12 @@ int prepare_passes( std::vector<pass_rt> &passes, int prec, tables *tb )
     {
         if ( p.iom )
         {
-            if ( !mixed_supported( p.N ) )
-                return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi: FP16-stored spectra need power-of-two lengths 128..2048, got " +
-                                                        std::to_string( p.N ) );
+            if ( !mixed_supported( p.N ) || ( ( p.iom & IO_SPLIT ) && p.N > 1024 ) )
+                return fail( FFTM_MI_NOT_SUPPORTED, ( p.iom & IO_SPLIT )
+                                                        ? "fftm_mi: hi + lo FP16-stored spectra need power-of-two "
+                                                          "lengths 128..1024, got " + std::to_string( p.N )
+                                                        : "fftm_mi: FP16-stored spectra need power-of-two lengths "
+                                                          "128..2048, got " + std::to_string( p.N ) );
             p.generic = false;
         }
         else if ( !p.plane )
@@ -725,7 +730,7 @@ void dense_line_strides( int ax, int dim, const int64_t *ext, const int64_t *so,
 // axis_pass with FP16 storage: xs (per logical axis) = exponent strides of the input
 // lines or null for FP32 input; ys = exponent strides of the output lines or null
 pass_rt axis_pass_h16( int kind, bool inv, int ax, int dim, const int64_t *ext, const int64_t *si, const int64_t *so,
-                       int src, int dst, int64_t N, const int64_t *xs, const int64_t *ys, int &rc )
+                       int src, int dst, int64_t N, const int64_t *xs, const int64_t *ys, int &rc, bool split = false )
 {
     pass_rt p;
     p.kind = kind;
@@ -736,7 +741,7 @@ pass_rt axis_pass_h16( int kind, bool inv, int ax, int dim, const int64_t *ext, 
     p.ies  = si[ax];
     p.oes  = so[ax];
     p.ax   = ax;
-    p.iom  = ( xs ? IO_H16_IN : 0 ) | ( ys ? IO_H16_OUT : 0 );
+    p.iom  = ( xs ? IO_H16_IN : 0 ) | ( ys ? IO_H16_OUT : 0 ) | ( split ? IO_SPLIT : 0 );
     p.xsn  = xs ? xs[ax] : 0;
     std::vector<bdim> v;
     for ( int k = 0; k < dim; ++k )
@@ -744,7 +749,8 @@ pass_rt axis_pass_h16( int kind, bool inv, int ax, int dim, const int64_t *ext, 
             v.push_back( bdim{ ext[k], si[k], so[k], xs ? xs[k] : 0, ys ? ys[k] : 0 } );
     rc = set_batch( p, v );
     // strided lines: twice the columns per workgroup (FP16 rows of 128 bytes), split exchange
-    p.var = p.loadb ? 7 : 0;
+    // (hi + lo storage: 8-byte bins, the split exchange at the default tile)
+    p.var = p.loadb ? ( split ? 6 : 7 ) : 0;
     return p;
 }
 
@@ -933,14 +939,14 @@ int build_ffts_passes( const ffts_layout &L, bool overwrite, std::vector<pass_rt
 // forward pass writes the FP32 spectrum; backward mirrors it.  Work area:
 // [FP16 spectrum: 4 bytes per bin][exponents E0][exponents E1], 256-byte aligned.
 int build_ffts_passes_h16( const ffts_layout &L, std::vector<pass_rt> &fwd, std::vector<pass_rt> &bwd,
-                           size_t &work_bytes )
+                           size_t &work_bytes, bool split = false )
 {
     if ( L.dim != 3 )
         return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi: FP16-stored spectra are implemented for the 3D layout" );
     const int64_t n0 = L.n[0], n1 = L.n[1], n2 = L.n[2], nzh = n2 / 2 + 1;
     const int64_t extc[3] = { n0, n1, nzh };
     auto          align   = []( size_t b ) { return ( b + 255 ) & ~(size_t)255; };
-    const size_t  hbytes  = align( (size_t)( n0 * n1 * nzh ) * 4 );
+    const size_t  hbytes  = align( (size_t)( n0 * n1 * nzh ) * ( split ? 8 : 4 ) );
     const size_t  e0bytes = align( (size_t)std::max( n0 * n1, n1 * nzh ) );
     const size_t  e1bytes = align( (size_t)( n0 * nzh ) );
     const int64_t E0 = (int64_t)hbytes, E1 = (int64_t)( hbytes + e0bytes );
@@ -953,29 +959,29 @@ int build_ffts_passes_h16( const ffts_layout &L, std::vector<pass_rt> &fwd, std:
     int rc = 0;
     fwd.clear();
     bwd.clear();
-    pass_rt p0 = axis_pass_h16( KIND_R2C, false, 2, 3, L.n, L.sr, L.sc, BUF_IN, BUF_WORK, n2, nullptr, yP0, rc );
+    pass_rt p0 = axis_pass_h16( KIND_R2C, false, 2, 3, L.n, L.sr, L.sc, BUF_IN, BUF_WORK, n2, nullptr, yP0, rc, split );
     p0.yoff    = E0;
     if ( rc )
         return rc;
-    pass_rt p1 = axis_pass_h16( KIND_C2C, false, 1, 3, extc, L.sc, L.sc, BUF_WORK, BUF_WORK, n1, yP0, yP1, rc );
+    pass_rt p1 = axis_pass_h16( KIND_C2C, false, 1, 3, extc, L.sc, L.sc, BUF_WORK, BUF_WORK, n1, yP0, yP1, rc, split );
     p1.xoff    = E0;
     p1.yoff    = E1;
     if ( rc )
         return rc;
-    pass_rt p2 = axis_pass_h16( KIND_C2C, false, 0, 3, extc, L.sc, L.sc, BUF_WORK, BUF_OUT, n0, yP1, nullptr, rc );
+    pass_rt p2 = axis_pass_h16( KIND_C2C, false, 0, 3, extc, L.sc, L.sc, BUF_WORK, BUF_OUT, n0, yP1, nullptr, rc, split );
     p2.xoff    = E1;
     if ( rc )
         return rc;
-    pass_rt q0 = axis_pass_h16( KIND_C2C, true, 0, 3, extc, L.sc, L.sc, BUF_IN, BUF_WORK, n0, nullptr, yQ0, rc );
+    pass_rt q0 = axis_pass_h16( KIND_C2C, true, 0, 3, extc, L.sc, L.sc, BUF_IN, BUF_WORK, n0, nullptr, yQ0, rc, split );
     q0.yoff    = E0;
     if ( rc )
         return rc;
-    pass_rt q1 = axis_pass_h16( KIND_C2C, true, 1, 3, extc, L.sc, L.sc, BUF_WORK, BUF_WORK, n1, yQ0, yQ1, rc );
+    pass_rt q1 = axis_pass_h16( KIND_C2C, true, 1, 3, extc, L.sc, L.sc, BUF_WORK, BUF_WORK, n1, yQ0, yQ1, rc, split );
     q1.xoff    = E0;
     q1.yoff    = E1;
     if ( rc )
         return rc;
-    pass_rt q2 = axis_pass_h16( KIND_C2R, true, 2, 3, L.n, L.sc, L.sr, BUF_WORK, BUF_OUT, n2, yQ1, nullptr, rc );
+    pass_rt q2 = axis_pass_h16( KIND_C2R, true, 2, 3, L.n, L.sc, L.sr, BUF_WORK, BUF_OUT, n2, yQ1, nullptr, rc, split );
     q2.xoff    = E1;
     if ( rc )
         return rc;
@@ -1004,6 +1010,7 @@ struct fftm_mi_ffts_s
     int32_t              layout = 0;
     int32_t              prec   = 0;     // arithmetic precision (FP32 for FFTM_MI_F32_H16)
     bool                 h16    = false; // FP16-stored intermediate spectra
+    bool                 h16x2  = false; // ... as hi + lo FP16 pairs (8 bytes per bin)
     bool                 fwd_work = false; // forward passes use the work area
     int32_t              flags  = 0;
     int64_t              n[4]   = { 0, 0, 0, 0 };
@@ -1381,7 +1388,8 @@ int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, 
     g_err.clear();
     if ( !n )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_validate: null sizes" );
-    if ( precision != FFTM_MI_F32 && precision != FFTM_MI_F64 && precision != FFTM_MI_F32_H16 )
+    if ( precision != FFTM_MI_F32 && precision != FFTM_MI_F64 && precision != FFTM_MI_F32_H16 &&
+         precision != FFTM_MI_F32_H16X2 )
         return fail( FFTM_MI_INVALID_TYPE, "fftm_mi_ffts_validate: unknown precision" );
     const int dim = layout == FFTM_MI_LAYOUT_3D ? 3 : 4;
     for ( int k = 0; k < dim; ++k )
@@ -1391,11 +1399,11 @@ int fftm_mi_ffts_validate( int32_t layout, const int64_t *n, int32_t precision, 
     int         rc = make_ffts_layout( layout, n, L );
     if ( rc != FFTM_MI_SUCCESS )
         return rc;
-    if ( precision == FFTM_MI_F32_H16 )
+    if ( precision == FFTM_MI_F32_H16 || precision == FFTM_MI_F32_H16X2 )
     {
         std::vector<pass_rt> f, b;
         size_t               wb = 0;
-        rc                      = build_ffts_passes_h16( L, f, b, wb );
+        rc                      = build_ffts_passes_h16( L, f, b, wb, precision == FFTM_MI_F32_H16X2 );
         if ( rc == FFTM_MI_SUCCESS )
             rc = prepare_passes( f, FFTM_MI_F32, nullptr );
         if ( rc == FFTM_MI_SUCCESS )
@@ -1434,7 +1442,8 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
     if ( !p )
         return fail( FFTM_MI_ALLOC_FAILED, "fftm_mi_ffts_create: out of host memory" );
     p->layout  = layout;
-    p->h16     = precision == FFTM_MI_F32_H16;
+    p->h16     = precision == FFTM_MI_F32_H16 || precision == FFTM_MI_F32_H16X2;
+    p->h16x2   = precision == FFTM_MI_F32_H16X2;
     p->prec    = p->h16 ? FFTM_MI_F32 : precision;
     p->flags   = flags;
     p->tb.prec = p->prec;
@@ -1446,7 +1455,7 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
     if ( p->h16 )
     {
         size_t wb = 0;
-        build_ffts_passes_h16( L, p->fwd, p->bwd, wb );
+        build_ffts_passes_h16( L, p->fwd, p->bwd, wb, p->h16x2 );
         p->fwd_work = true;
     }
     else
@@ -1565,7 +1574,8 @@ int fftm_mi_ffts_pass_bytes( fftm_mi_ffts h, int32_t backward, int32_t index, in
     {
         // FP16 side: 4 bytes per bin; exponents: one byte per stored line
         const int64_t lines = p.kind == KIND_C2C ? p.ncols : p.ncols;
-        const int64_t ic = ( p.iom & IO_H16_IN ) ? 4 : cs, oc = ( p.iom & IO_H16_OUT ) ? 4 : cs;
+        const int64_t hs = ( p.iom & IO_SPLIT ) ? 8 : 4; // bytes per FP16-stored bin
+        const int64_t ic = ( p.iom & IO_H16_IN ) ? hs : cs, oc = ( p.iom & IO_H16_OUT ) ? hs : cs;
         const int64_t ex = ( ( p.iom & IO_H16_IN ) ? lines : 0 ) + ( ( p.iom & IO_H16_OUT ) ? lines : 0 );
         if ( p.kind == KIND_C2C )
             *bytes = p.ncols * p.N * ( ic + oc ) + ex;
@@ -2010,7 +2020,7 @@ std::string problem_key( const fftm_mi_ffts_s &h )
     o << "FFTM_MI_TILE_" << names[h.layout] << "_" << h.n[0] << "x" << h.n[1] << "x" << h.n[2];
     if ( h.layout != FFTM_MI_LAYOUT_3D )
         o << "x" << h.n[3];
-    o << ( h.h16 ? "_F32_H16" : ( h.prec == FFTM_MI_F32 ? "_F32" : "_F64" ) );
+    o << ( h.h16x2 ? "_F32_H16X2" : ( h.h16 ? "_F32_H16" : ( h.prec == FFTM_MI_F32 ? "_F32" : "_F64" ) ) );
     // creation flags change the pass list (plane fusion, in-place backward): records are
     // per pass index, so a handle only takes records measured with the same flags
     if ( h.flags )
@@ -2317,7 +2327,10 @@ int fftm_mi_ffts_autotune( fftm_mi_ffts h, const void *real_in, void *spectrum_o
         if ( e1 )
             (void)hipEventDestroy( e1 );
     };
-    if ( hipMalloc( &real_out, rbytes ) != hipSuccess || hipMalloc( &ref, std::max( rbytes, cbytes ) ) != hipSuccess ||
+    // ref holds a copy of any pass's whole destination: the real output, the spectrum or the
+    // work area (which is larger than the spectrum with hi + lo FP16 storage)
+    const size_t refbytes = std::max( std::max( rbytes, cbytes ), h->work_size );
+    if ( hipMalloc( &real_out, rbytes ) != hipSuccess || hipMalloc( &ref, refbytes ) != hipSuccess ||
          hipMalloc( &dcount, sizeof( unsigned long long ) ) != hipSuccess || hipEventCreate( &e0 ) != hipSuccess ||
          hipEventCreate( &e1 ) != hipSuccess )
     {

@@ -120,9 +120,25 @@ int fixed_tile_cols_f64( int N, bool loadb, int var )
 
 FFTM_MI_MIXED_LENGTHS( FFTM_MI_DECLARE_MIXED )
 
+#define FFTM_MI_MIXED_SPLIT_LENGTHS( X ) X( 128 ) X( 256 ) X( 512 ) X( 1024 )
+#define FFTM_MI_DECLARE_MIXED_SPLIT( N )                                                                               \
+    hipError_t launch_mixed_split_f32_n##N( const pass_params<float> &, int, bool, bool, int, int, hipStream_t );
+#define FFTM_MI_CASE_MIXED_SPLIT( N )                                                                                  \
+    case N: return launch_mixed_split_f32_n##N( p, kind, inv, loadb, var, iom, s );
+
+FFTM_MI_MIXED_SPLIT_LENGTHS( FFTM_MI_DECLARE_MIXED_SPLIT )
+
 hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, bool inv, bool loadb, int var, int iom,
                                   hipStream_t s )
 {
+    if ( iom & IO_SPLIT )
+    {
+        switch ( N )
+        {
+            FFTM_MI_MIXED_SPLIT_LENGTHS( FFTM_MI_CASE_MIXED_SPLIT )
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch ( N )
     {
         FFTM_MI_MIXED_LENGTHS( FFTM_MI_CASE_MIXED )

@@ -4,6 +4,8 @@
 
 #include "mi_launch.hpp"
 
+#include <type_traits>
+
 namespace fftm_mi
 {
 namespace detail
@@ -135,13 +137,13 @@ hipError_t launch_mixed_io( const pass_params<float> &p, int kind, bool inv, boo
         return loadb ? launch_fixed<float, N, KIND_C2C, false, true, VAR, IOM>( p, s )
                      : launch_fixed<float, N, KIND_C2C, false, false, VAR, IOM>( p, s );
     }
-    if constexpr ( IOM == IO_H16_OUT )
+    if constexpr ( ( IOM & ~IO_SPLIT ) == IO_H16_OUT )
     {
         if ( kind == KIND_R2C )
             return loadb ? launch_fixed<float, N, KIND_R2C, false, true, VAR, IOM>( p, s )
                          : launch_fixed<float, N, KIND_R2C, false, false, VAR, IOM>( p, s );
     }
-    if constexpr ( IOM == IO_H16_IN )
+    if constexpr ( ( IOM & ~IO_SPLIT ) == IO_H16_IN )
     {
         if ( kind == KIND_C2R )
             return loadb ? launch_fixed<float, N, KIND_C2R, true, true, VAR, IOM>( p, s )
@@ -164,6 +166,29 @@ hipError_t launch_mixed_v( const pass_params<float> &p, int kind, bool inv, bool
 
 // tile-shape variants compiled for the FP16-storage kernels
 constexpr bool mixed_variant( int var ) { return var == 0 || var == 6 || var == 7; }
+
+// hi + lo FP16-stored spectra (IO_SPLIT): tile shapes 0 and 6
+template <int N>
+hipError_t launch_mixed_split_n( const pass_params<float> &p, int kind, bool inv, bool loadb, int var, int iom,
+                                 hipStream_t s )
+{
+    auto go = [&]( auto vtag ) -> hipError_t {
+        constexpr int V = decltype( vtag )::value;
+        switch ( iom )
+        {
+        case IO_H16_IN | IO_SPLIT: return launch_mixed_io<N, V, IO_H16_IN | IO_SPLIT>( p, kind, inv, loadb, s );
+        case IO_H16_OUT | IO_SPLIT: return launch_mixed_io<N, V, IO_H16_OUT | IO_SPLIT>( p, kind, inv, loadb, s );
+        case IO_H16_IN | IO_H16_OUT | IO_SPLIT:
+            return launch_mixed_io<N, V, IO_H16_IN | IO_H16_OUT | IO_SPLIT>( p, kind, inv, loadb, s );
+        default: return hipErrorInvalidValue;
+        }
+    };
+    if ( var == 0 )
+        return go( std::integral_constant<int, 0>{} );
+    if ( var == 6 )
+        return go( std::integral_constant<int, 6>{} );
+    return hipErrorInvalidValue;
+}
 
 template <int N>
 hipError_t launch_mixed_n( const pass_params<float> &p, int kind, bool inv, bool loadb, int var, int iom, hipStream_t s )
@@ -263,6 +288,17 @@ hipError_t launch_rs_n( const pass_params<float> &p, bool inv, int shape, int rs
                                       hipStream_t s )                                                                  \
     {                                                                                                                  \
         return detail::launch_mixed_n<N>( p, kind, inv, loadb, var, iom, s );                                          \
+    }                                                                                                                  \
+    }
+
+// One length N of the hi + lo FP16-storage kernels: defines launch_mixed_split_f32_n<N>.
+#define FFTM_MI_DEFINE_MIXED_SPLIT_LENGTH( N )                                                                         \
+    namespace fftm_mi                                                                                                  \
+    {                                                                                                                  \
+    hipError_t launch_mixed_split_f32_n##N( const pass_params<float> &p, int kind, bool inv, bool loadb, int var,      \
+                                            int iom, hipStream_t s )                                                   \
+    {                                                                                                                  \
+        return detail::launch_mixed_split_n<N>( p, kind, inv, loadb, var, iom, s );                                    \
     }                                                                                                                  \
     }
 

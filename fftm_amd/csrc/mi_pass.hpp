@@ -68,13 +68,17 @@ struct pass_params
 };
 
 typedef _Float16 h16x2 __attribute__( ( ext_vector_type( 2 ) ) );
+typedef _Float16 h16x4 __attribute__( ( ext_vector_type( 4 ) ) );
 
 // storage modes of a pass (template parameter IOM): bit 0 FP16 input, bit 1 FP16 output
 enum : int
 {
     IO_F32    = 0,
     IO_H16_IN = 1,
-    IO_H16_OUT = 2
+    IO_H16_OUT = 2,
+    // with IO_H16_IN / IO_H16_OUT: every FP16-stored bin is a hi + lo pair (8 bytes,
+    // {hi.re, hi.im, lo.re, lo.im}, value = (hi + lo 2^-11) 2^e): FP32-level precision
+    IO_SPLIT = 4
 };
 
 // two consecutive elements of T (8 bytes for float, 16 for double) and two consecutive
@@ -456,12 +460,34 @@ __device__ __forceinline__ void col_exponents( const pass_params<T> &p, int64_t 
     yo               = c0 * p.ys0 + c1 * p.ys1 + c2 * p.ys2;
 }
 
-// FP16 element * 2^e in FP32
+// storage element of an FP16-stored bin: one FP16 pair, or a hi + lo pair of pairs
+template <bool SPLIT>
+struct h16_elem
+{
+    typedef h16x2 type;
+};
+template <>
+struct h16_elem<true>
+{
+    typedef h16x4 type;
+};
+
+// FP16-stored bin * 2^e in FP32
+template <bool SPLIT>
+__device__ __forceinline__ cpx<float> h16_decode( typename h16_elem<SPLIT>::type h, int e )
+{
+    if constexpr ( SPLIT )
+        return mk<float>( __builtin_ldexpf( (float)h.x + (float)h.z * ( 1.0f / 2048.0f ), e ),
+                          __builtin_ldexpf( (float)h.y + (float)h.w * ( 1.0f / 2048.0f ), e ) );
+    else
+        return mk<float>( __builtin_ldexpf( (float)h.x, e ), __builtin_ldexpf( (float)h.y, e ) );
+}
+
+template <bool SPLIT>
 __device__ __forceinline__ cpx<float> h16_load( const void *base, int64_t idx, const int8_t *xp, int64_t xidx )
 {
-    const h16x2 h = static_cast<const h16x2 *>( base )[idx];
-    const int   e = xp[xidx];
-    return mk<float>( __builtin_ldexpf( (float)h.x, e ), __builtin_ldexpf( (float)h.y, e ) );
+    using E = typename h16_elem<SPLIT>::type;
+    return h16_decode<SPLIT>( static_cast<const E *>( base )[idx], xp[xidx] );
 }
 
 // exponent of a line whose largest component magnitude is amax: the stored values
@@ -474,14 +500,22 @@ __device__ __forceinline__ int h16_exponent( float amax )
     return e < -127 ? -127 : ( e > 127 ? 127 : e );
 }
 
-__device__ __forceinline__ h16x2 h16_pack( float a, float b, int e )
+// (a, b) 2^-e rounded to FP16; with SPLIT also the rounding residuals times 2^11 (at most
+// half an FP16 ulp of a value below 2^15, i.e. 8, so the scaled residual stays below 2^14)
+template <bool SPLIT>
+__device__ __forceinline__ typename h16_elem<SPLIT>::type h16_pack( float a, float b, int e )
 {
-    return h16x2{ (_Float16)__builtin_ldexpf( a, -e ), (_Float16)__builtin_ldexpf( b, -e ) };
+    const float    sa = __builtin_ldexpf( a, -e ), sb = __builtin_ldexpf( b, -e );
+    const _Float16 ha = (_Float16)sa, hb = (_Float16)sb;
+    if constexpr ( SPLIT )
+        return h16x4{ ha, hb, (_Float16)( ( sa - (float)ha ) * 2048.0f ), (_Float16)( ( sb - (float)hb ) * 2048.0f ) };
+    else
+        return h16x2{ ha, hb };
 }
 
 // element n of an FP16-stored input line (C2C) or of the Hermitian extension of two
 // FP16-stored half spectra (C2R), as in load_line
-template <int KIND>
+template <int KIND, bool SPLIT>
 __device__ __forceinline__ cpx<float> load_line_h16( const pass_params<float> &p, int64_t ia, int64_t ib, int64_t xa,
                                                      int64_t xb, bool va, bool vb, int n, int N )
 {
@@ -490,15 +524,15 @@ __device__ __forceinline__ cpx<float> load_line_h16( const pass_params<float> &p
     // join, one memory latency per element
     if constexpr ( KIND == KIND_C2C )
     {
-        const cpx<float> v = h16_load( p.in, va ? ia + (int64_t)n * p.ies : 0, p.xin, va ? xa + (int64_t)n * p.xsn : 0 );
+        const cpx<float> v = h16_load<SPLIT>( p.in, va ? ia + (int64_t)n * p.ies : 0, p.xin, va ? xa + (int64_t)n * p.xsn : 0 );
         return va ? v : mk<float>( 0, 0 );
     }
     else
     {
         const int  h  = N / 2;
         const int  nn = n <= h ? n : N - n;
-        cpx<float> a  = h16_load( p.in, va ? ia + (int64_t)nn * p.ies : 0, p.xin, va ? xa + (int64_t)nn * p.xsn : 0 );
-        cpx<float> b  = h16_load( p.in, vb ? ib + (int64_t)nn * p.ies : 0, p.xin, vb ? xb + (int64_t)nn * p.xsn : 0 );
+        cpx<float> a  = h16_load<SPLIT>( p.in, va ? ia + (int64_t)nn * p.ies : 0, p.xin, va ? xa + (int64_t)nn * p.xsn : 0 );
+        cpx<float> b  = h16_load<SPLIT>( p.in, vb ? ib + (int64_t)nn * p.ies : 0, p.xin, vb ? xb + (int64_t)nn * p.xsn : 0 );
         if ( !va )
             a = mk<float>( 0, 0 );
         if ( !vb )
@@ -603,6 +637,9 @@ struct pass_kernel
     static_assert( !FLT || ( KIND == KIND_C2C && !INV && IOM == IO_F32 && !variant_split( VAR ) ),
                    "the fused filter pass is a forward C2C pass with whole-value LDS exchange" );
     static_assert( IOM == IO_F32 || std::is_same<T, float>::value, "FP16 storage is for FP32 transforms" );
+    static_assert( IOM != IO_SPLIT, "hi + lo storage qualifies an FP16-stored side" );
+    static constexpr bool SPL = ( IOM & IO_SPLIT ) != 0;
+    using HE                  = typename h16_elem<SPL>::type;
     static_assert( !( IOM & IO_H16_IN ) || KIND != KIND_R2C, "R2C reads real FP32 input" );
     static_assert( !( IOM & IO_H16_OUT ) || KIND != KIND_C2R, "C2R writes real FP32 output" );
     using C                    = cpx<T>;
@@ -825,7 +862,7 @@ struct pass_kernel
                     }
                 }
                 const int e = h16_exponent( column_max( red, lm, x.c, LOADB ) * fabsf( (float)p.scale ) );
-                h16x2    *dst = static_cast<h16x2 *>( p.out );
+                HE       *dst = static_cast<HE *>( p.out );
 #pragma unroll
                 for ( int b = 0; b < NB; ++b )
                 {
@@ -836,7 +873,7 @@ struct pass_kernel
 #pragma unroll
                         for ( int m = 0; m < R; ++m )
                             dst[x.oa + (int64_t)( d + m * NS ) * p.oes] =
-                                h16_pack( v[b][m].x * p.scale, v[b][m].y * p.scale, e );
+                                h16_pack<SPL>( v[b][m].x * p.scale, v[b][m].y * p.scale, e );
                     }
                 }
                 if ( x.t == 0 && x.va )
@@ -968,7 +1005,7 @@ struct pass_kernel
                 int64_t   xi, ya, yb = 0;
                 col_exponents( p, col, xi, ya );
                 col_exponents( p, col + 1, xi, yb );
-                h16x2 *out = static_cast<h16x2 *>( p.out );
+                HE *out = static_cast<HE *>( p.out );
 #pragma unroll
                 for ( int i = 0; i < NI; ++i )
                 {
@@ -976,9 +1013,9 @@ struct pass_kernel
                     if ( LIM % TPC == 0 || k < LIM )
                     {
                         if ( va )
-                            out[oa + (int64_t)k * p.oes] = h16_pack( ra[i].x, ra[i].y, ea );
+                            out[oa + (int64_t)k * p.oes] = h16_pack<SPL>( ra[i].x, ra[i].y, ea );
                         if ( vb )
-                            out[ob + (int64_t)k * p.oes] = h16_pack( rb[i].x, rb[i].y, eb );
+                            out[ob + (int64_t)k * p.oes] = h16_pack<SPL>( rb[i].x, rb[i].y, eb );
                     }
                 }
                 if ( t2 == 0 )
@@ -1018,13 +1055,13 @@ struct pass_kernel
             const int e = h16_exponent( column_max( red, lm, c2, p.store_b != 0 ) * fabsf( (float)p.scale ) );
             int64_t   xi, ya;
             col_exponents( p, col, xi, ya );
-            h16x2 *out = static_cast<h16x2 *>( p.out );
+            HE *out = static_cast<HE *>( p.out );
 #pragma unroll
             for ( int i = 0; i < NI; ++i )
             {
                 const int n = t2 + i * TPC;
                 if ( ( LIM % TPC == 0 || n < LIM ) && va )
-                    out[oa + (int64_t)n * p.oes] = h16_pack( zk[i].x * p.scale, zk[i].y * p.scale, e );
+                    out[oa + (int64_t)n * p.oes] = h16_pack<SPL>( zk[i].x * p.scale, zk[i].y * p.scale, e );
             }
             if ( t2 == 0 && va )
                 p.xout[ya] = (int8_t)e;
@@ -1217,7 +1254,7 @@ struct pass_kernel
                 const int64_t src  = ok ? lo + (int64_t)k * p.ies : 0;
                 C             z;
                 if constexpr ( IOM & IO_H16_IN )
-                    z = h16_load( p.in, src, p.xin, ok ? xoff[line] + (int64_t)k * p.xsn : 0 );
+                    z = h16_load<SPL>( p.in, src, p.xin, ok ? xoff[line] + (int64_t)k * p.xsn : 0 );
                 else
                     z = static_cast<const C *>( p.in )[src];
                 zs[i] = ok ? z : mk<T>( 0, 0 );
@@ -1333,8 +1370,9 @@ struct pass_kernel
                     const int ec = ( 2 * N ) % NT == 0 || e < 2 * N ? e : 0;
                     er[i]        = p.xin[( ec < N ? xf : xl ) + (int64_t)( ec % N ) * p.xsn];
                 }
-                // the raw FP16 pair waits in the real component's register
-                const h16x2 *src = static_cast<const h16x2 *>( p.in );
+                // the raw stored bin waits in the value's registers (an FP16 pair in the real
+                // component's, a hi + lo pair in both)
+                const HE *src = static_cast<const HE *>( p.in );
 #pragma unroll
                 for ( int b = 0; b < NB0; ++b )
                 {
@@ -1342,7 +1380,13 @@ struct pass_kernel
                     const bool ok = x.va && ( NBF0 % TPC == 0 || j < NBF0 );
 #pragma unroll
                     for ( int r = 0; r < R0; ++r )
-                        v[b][r].x = __builtin_bit_cast( float, src[ok ? x.ia + (int64_t)( j + r * NBF0 ) * p.ies : 0] );
+                    {
+                        const HE raw = src[ok ? x.ia + (int64_t)( j + r * NBF0 ) * p.ies : 0];
+                        if constexpr ( SPL )
+                            v[b][r] = __builtin_bit_cast( C, raw );
+                        else
+                            v[b][r].x = __builtin_bit_cast( float, raw );
+                    }
                 }
 #pragma unroll
                 for ( int i = 0; i < EP; ++i )
@@ -1362,11 +1406,13 @@ struct pass_kernel
 #pragma unroll
                         for ( int r = 0; r < R0; ++r )
                         {
-                            const int   e = xe[g + j + r * NBF0];
-                            const h16x2 h = __builtin_bit_cast( h16x2, v[b][r].x );
-                            v[b][r]       = x.va ? mk<float>( __builtin_ldexpf( (float)h.x, e ),
-                                                              __builtin_ldexpf( (float)h.y, e ) )
-                                                 : mk<float>( 0, 0 );
+                            const int e = xe[g + j + r * NBF0];
+                            HE        h;
+                            if constexpr ( SPL )
+                                h = __builtin_bit_cast( HE, v[b][r] );
+                            else
+                                h = __builtin_bit_cast( HE, v[b][r].x );
+                            v[b][r] = x.va ? h16_decode<SPL>( h, e ) : mk<float>( 0, 0 );
                         }
                         dft_fixed<R0, INV>( v[b], p.tw, N );
                     }
@@ -1386,7 +1432,7 @@ struct pass_kernel
                     for ( int r = 0; r < R0; ++r )
                     {
                         if constexpr ( IOM & IO_H16_IN )
-                            v[b][r] = load_line_h16<KIND>( p, x.ia, x.ib, x.xa, x.xb, x.va, x.vb, j + r * NBF0, N );
+                            v[b][r] = load_line_h16<KIND, SPL>( p, x.ia, x.ib, x.xa, x.xb, x.va, x.vb, j + r * NBF0, N );
                         else
                             v[b][r] = load_line<KIND>( p, x.ia, x.ib, x.va, x.vb, j + r * NBF0, N );
                     }

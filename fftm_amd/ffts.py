@@ -96,7 +96,8 @@ class ffts_init_options:
     destructive_backward: bool = False
     # "fp32": spectra between passes in the transform precision; "fp16": FP32 transforms
     # whose intermediate half spectra are stored in FP16 with per-line power-of-two
-    # exponents (3D, fused execution, lengths 128..2048)
+    # exponents (3D, fused execution, lengths 128..2048); "fp16x2": the same with every
+    # stored bin a hi + lo FP16 pair (FP32-level precision, FP32's bytes; lengths 128..1024)
     spectrum_storage: str = "fp32"
 
 
@@ -182,7 +183,7 @@ class ffts:
         for d in dims:
             if int(d) < 1:
                 raise ValueError("ffts::init: sizes must be positive")
-        if self._options.spectrum_storage == "fp16" and (dim != 3 or self._options.execution != "fused"):
+        if self._options.spectrum_storage in ("fp16", "fp16x2") and (dim != 3 or self._options.execution != "fused"):
             raise ValueError("ffts: FP16 spectrum storage is implemented for fused 3D transforms")
         with self._profiler.scope(f"ffts::init<{dim}>"):
             self.nx, self.ny = int(nx), int(ny)
@@ -226,12 +227,12 @@ class ffts:
         import torch
         L = _lib.lib()
         prec = _lib.F32 if self.real == torch.float32 else _lib.F64
-        if self._options.spectrum_storage == "fp16":
+        if self._options.spectrum_storage in ("fp16", "fp16x2"):
             if self.real != torch.float32:
                 raise TypeError("ffts: FP16 spectrum storage needs float32 transforms")
-            prec = _lib.F32_H16
+            prec = _lib.F32_H16 if self._options.spectrum_storage == "fp16" else _lib.F32_H16X2
         elif self._options.spectrum_storage != "fp32":
-            raise ValueError("ffts: spectrum_storage is 'fp32' or 'fp16'")
+            raise ValueError("ffts: spectrum_storage is 'fp32', 'fp16' or 'fp16x2'")
         flags = _lib.FLAG_OVERWRITE_INPUT if self._options.destructive_backward else 0
         h = ctypes.c_void_p()
         _lib.check(L.fftm_mi_ffts_create(layout, _lib.i64_array(n), prec, flags, ctypes.byref(h)),

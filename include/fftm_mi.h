@@ -57,6 +57,9 @@ extern "C" {
  * (4 bytes per bin instead of 8).  ffts 3D layout only; lengths 128..2048 (powers of
  * two).  No counterpart in the reference.                                             */
 #define FFTM_MI_F32_H16 2
+/* As FFTM_MI_F32_H16, with every stored bin a hi + lo pair of FP16 pairs (8 bytes, value
+ * (hi + lo 2^-11) 2^e): FP32-level precision at FP32's bytes.  Lengths 128..1024.      */
+#define FFTM_MI_F32_H16X2 3
 
 /* --- plan flags ----------------------------------------------------------- */
 /* C2R of rank >= 2 keeps its input intact by default and needs a work area of one

@@ -238,3 +238,81 @@ def test_h16_adjoint_without_work_area_is_refused(gpu):
             _lib.NO_WORKSPACE
     finally:
         L.fftm_mi_ffts_destroy(h)
+
+
+# hi + lo FP16 storage (FFTM_MI_F32_H16X2): each stored bin keeps hi = fp16(v) and
+# lo = fp16((v - hi) 2^11), about 22 significant bits, so the FP32 parity bar applies
+TOL_X2_L2 = 2e-6
+TOL_X2_MAX = 1e-5
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128), (256, 128, 512), (512, 512, 128)])
+def test_h16x2_matches_oracle(gpu, dims):
+    import torch
+    print(f"case h16x2 3d {dims}", flush=True)
+    x = oracle.fill_random_real_3d(*dims, 0x1234ABCDEF987654, np.float32)
+    f = _ffts(dims, "fp16x2")
+    u = torch.from_numpy(x).cuda()
+    X = f.complex_array()
+    f.forward(u, X)
+    Xo = oracle.ffts3d_forward(x.astype(np.float64))
+    l2, mx = oracle.rel_errors(from_gpu(X), Xo)
+    print(f"[h16x2] fwd {dims}: rel_l2={l2:.3e} rel_max={mx:.3e}", flush=True)
+    assert l2 < TOL_X2_L2 and mx < TOL_X2_MAX
+    y = f.real_array()
+    f.backward(X, y)
+    yr = from_gpu(y).astype(np.float64) / np.prod(dims)
+    rt = float(np.sqrt(np.mean((yr - x) ** 2)))
+    print(f"[h16x2] round trip {dims}: rms={rt:.3e}", flush=True)
+    assert rt < 10 * float(np.finfo(np.float32).eps)
+    f.close()
+
+
+@pytest.mark.parametrize("shift", [20, -30])
+def test_h16x2_power_of_two_scaling_is_exact(gpu, shift):
+    import torch
+    dims = (128, 128, 256)
+    x = oracle.fill_random_real_3d(*dims, 77, np.float32)
+    f = _ffts(dims, "fp16x2")
+    u = torch.from_numpy(x).cuda()
+    X, Xs = f.complex_array(), f.complex_array()
+    f.forward(u, X)
+    f.forward(u * (2.0 ** shift), Xs)
+    assert torch.equal(Xs, X * (2.0 ** shift))
+    f.close()
+
+
+def test_h16x2_variants_and_limits(gpu):
+    """Tile shapes 0 and 6 (each in every tile order) give bitwise the same result; shape 7,
+    the alternative radix schedules and lengths above 1024 are not compiled for it."""
+    import torch
+    import fftm_amd
+    from fftm_amd import _lib
+    L = _lib.lib()
+    dims = (256, 256, 256)
+    f = _ffts(dims, "fp16x2")
+    u = torch.rand(f.real_shape(), dtype=torch.float32, device="cuda") * 2 - 1
+    X0, y0 = f.complex_array(), f.real_array()
+    f.forward(u, X0)
+    f.backward(X0, y0)
+    nf, nb = f.num_passes()
+    for var in (0, 6, 8, 14, 16, 22, 24, 30):
+        for bw, n in ((0, nf), (1, nb)):
+            for i in range(n):
+                _lib.check(L.fftm_mi_ffts_set_pass_variant(f._handle, bw, i, var), "set_pass_variant")
+        X, y = f.complex_array(), f.real_array()
+        f.forward(u, X)
+        f.backward(X, y)
+        assert torch.equal(X, X0) and torch.equal(y, y0), f"variant {var}"
+    assert L.fftm_mi_ffts_set_pass_variant(f._handle, 0, 1, 7) == _lib.NOT_SUPPORTED
+    # the tuner (its reference copy of a pass output covers the 8-byte-per-bin work area)
+    chosen = f.autotune(warmup=1, iterations=2)
+    assert all((v & 7) in (0, 6) and v < 32 for v in chosen.values())
+    X, y = f.complex_array(), f.real_array()
+    f.forward(u, X)
+    f.backward(X, y)
+    assert torch.equal(X, X0) and torch.equal(y, y0)
+    f.close()
+    import ctypes
+    ws = ctypes.c_size_t()
+    assert L.fftm_mi_ffts_validate(0, _lib.i64_array([2048, 128, 128]), _lib.F32_H16X2, 0, ctypes.byref(ws)) == 16
