@@ -194,3 +194,35 @@ def test_filter_argument_errors(gpu):
     assert e.value.status == 16
     f.close()
     g.close()
+
+
+@pytest.mark.parametrize("fvar", [0, 2, 4, 28])
+def test_filter_pass_tile_shapes_bitwise(gpu, tmp_path, fvar):
+    """Every tile shape of the fused filter pass (0, 2, 4, in any tile order) gives bitwise
+    the result of forward + spectral_apply + backward.  The shape is installed through a
+    tuning cache holding the handle's filter record."""
+    import torch
+    import fftm_amd
+    dims = (512, 64, 32)
+    f = fftm_amd.ffts("float32")
+    f.init(*dims, options=fftm_amd.ffts_init_options(profiling_key="", memory_profiling_key=""))
+    cache = str(tmp_path / "flt.cache")
+    f.autotune(warmup=0, iterations=1, cache_path=cache)
+    lines = open(cache).read().splitlines()
+    key = [ln.split("=", 1)[0] for ln in lines if ln.endswith(tuple(f"={v}" for v in range(32)))
+           and ln.split("=", 1)[0].endswith("_FILTER")]
+    assert len(key) == 1, key
+    with open(cache, "w") as fh:
+        fh.write("\n".join(ln if not ln.startswith(key[0] + "=") else f"{key[0]}={fvar}" for ln in lines) + "\n")
+    assert f.load_tuning(cache) > 0
+    g = torch.Generator(device="cuda").manual_seed(fvar)
+    u = torch.rand(dims, generator=g, device="cuda") * 2 - 1
+    out = f.real_array()
+    f.filter(u, out, "gaussian", scale=1.0 / u.numel(), sigma=6.0)
+    X = f.complex_array()
+    f.forward(u, X)
+    f.spectral_apply(X, X, "gaussian", scale=1.0 / u.numel(), sigma=6.0)
+    y = f.real_array()
+    f.backward(X, y)
+    assert torch.equal(y, out), f"filter variant {fvar}"
+    f.close()
