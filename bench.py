@@ -51,16 +51,82 @@ def _b_alg(dims, elem_real):
     return 2 * ((elem_real * N + 2 * elem_real * Nc) + (d - 1) * 4 * elem_real * Nc)
 
 
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def source_stamp():
+    """Digest of the kernel sources (fftm_amd/csrc/*.hip, *.hpp): a traffic figure measured
+    on another build of the kernels is stale."""
+    import hashlib
+    csrc = os.path.join(ROOT, "fftm_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hip", ".hpp")):
+            h.update(name.encode())
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _traffic_summary(kernel_key):
-    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    """(HBM bytes per launch measured with rocprofv3 --pmc, stamp of the build they were
+    measured on); the bytes are None when the committed summary lacks the pass or was
+    measured on other kernel sources (stale)."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_PATH) as f:
             data = json.load(f)
-        v = data.get(kernel_key)
-        return None if v is None else float(v["bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
-        return None
+    except (OSError, ValueError):
+        return None, None
+    stamp = data.get("source_stamp")
+    v = data.get("passes", {}).get(kernel_key)
+    if v is None or stamp != source_stamp():
+        return None, stamp
+    return float(v["bytes_per_launch"]), stamp
+
+
+def write_traffic(trace_csv, fetch_csv, write_csv, passes=6, pairs=30):
+    """profiles/traffic.json from the rocprofv3 runs of bench.py --no-cpu --no-check --no-4d
+    --no-extra --calibrate-copy --tuning-cache C (trace, --pmc FETCH_SIZE, --pmc WRITE_SIZE):
+    the last pairs * passes k_pass dispatches are the timed loops, pass = position mod 6;
+    the counters are scaled by their reading for the 1 GiB copy (read 2^30 B, written 2^30 B)."""
+    import csv
+
+    def rows(path, counter=None):
+        out = []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if counter is not None and r.get("Counter_Name") != counter:
+                    continue
+                out.append(r)
+        return out
+
+    def per_dispatch(path, counter):
+        acc = {}
+        for r in rows(path, counter):
+            acc.setdefault(int(r["Dispatch_Id"]), [r["Kernel_Name"], 0.0])[1] += float(r["Counter_Value"])
+        return [acc[k] for k in sorted(acc)]
+
+    def calib(seq):
+        vals = [v for name, v in seq if "copyBuffer" in name]
+        return (1 << 30) / (max(vals) * 1024.0) if vals else 1.0
+
+    fetch, write = per_dispatch(fetch_csv, "FETCH_SIZE"), per_dispatch(write_csv, "WRITE_SIZE")
+    fr, fw = calib(fetch), calib(write)
+    fk = [v * 1024.0 * fr for name, v in fetch if "k_pass" in name][-pairs * passes:]
+    wk = [v * 1024.0 * fw for name, v in write if "k_pass" in name][-pairs * passes:]
+    tr = [r for r in rows(trace_csv) if "k_pass" in r["Kernel_Name"]][-pairs * passes:]
+    names = [f"ffts3d_512_f32_{'bwd' if k >= 3 else 'fwd'}{k % 3}" for k in range(passes)]
+    out = {"source_stamp": source_stamp(), "fetch_scale": fr, "write_scale": fw, "passes": {}}
+    for k, name in enumerate(names):
+        rd = [fk[i] for i in range(k, len(fk), passes)]
+        wr = [wk[i] for i in range(k, len(wk), passes)]
+        us = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in tr[k::passes]]
+        out["passes"][name] = {"bytes_per_launch": round(float(np.mean(rd)) + float(np.mean(wr)), 1),
+                               "read": round(float(np.mean(rd)), 1), "write": round(float(np.mean(wr)), 1),
+                               "avg_us": round(float(np.mean(us)), 3), "kernel": tr[k]["Kernel_Name"][:120]}
+    with open(TRAFFIC_PATH, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
 
 
 def fill_field(u, dims, pencil=False):
@@ -621,7 +687,13 @@ def main():
                     help="skip the 1024^3 FP32 vs FP16-storage comparison (the mixed-precision workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy 5 times (counter calibration)")
+    ap.add_argument("--write-traffic", nargs=3, metavar=("TRACE_CSV", "FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from rocprofv3 kernel-trace / FETCH_SIZE / WRITE_SIZE csv files "
+                         "of a run with --no-cpu --no-check --no-4d --no-extra --calibrate-copy, then exit")
     args = ap.parse_args()
+    if args.write_traffic:
+        print(json.dumps(write_traffic(*args.write_traffic, pairs=3 * args.steps), indent=1))
+        return
     global TUNING_CACHE
     TUNING_CACHE = args.tuning_cache
     if args.gpus != 1:
@@ -663,7 +735,7 @@ def main():
     dom_bytes = r3["pass_bytes"][dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     dom_name = f"{'backward' if dom[0] else 'forward'} pass {dom[1]}"
-    traffic = _traffic_summary(f"ffts3d_{n}_f32_{'bwd' if dom[0] else 'fwd'}{dom[1]}")
+    traffic, traffic_stamp = _traffic_summary(f"ffts3d_{n}_f32_{'bwd' if dom[0] else 'fwd'}{dom[1]}")
 
     out = {
         "metric": "R2C+C2R GFLOP/s & HBM GB/s at 512^3/64^4, mixed vs FP32 at ≤1e-5 RMS err",
@@ -688,7 +760,10 @@ def main():
         "pass_variants": r3["variants"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "avg_ms": round(dom_ms, 4)},
+                     "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "avg_ms": round(dom_ms, 4),
+                     "traffic_source": {"file": "profiles/traffic.json", "measured_on": traffic_stamp,
+                                        "this_build": source_stamp(),
+                                        "stale": traffic_stamp is not None and traffic_stamp != source_stamp()}},
     }
     if not args.no_check:
         out["errors_vs_fp64"] = {k: r3[k] for k in ("fwd_rel_l2_vs_fp64", "fwd_rel_max_vs_fp64",
