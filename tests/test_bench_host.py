@@ -55,3 +55,21 @@ def test_cpu_baseline_protocol_small():
         assert leg["pair_ms"]["min"] <= leg["pair_ms"]["median"] <= leg["pair_ms"]["max"]
         assert leg["roundtrip_rel_l2"] < 10 * eps
     assert r["value"] == r["f64"]["gflops"] > 0
+
+
+def test_traffic_figure_is_stamped_with_the_kernel_sources(tmp_path, monkeypatch):
+    """roofline.traffic comes from profiles/traffic.json only when the file was measured
+    on the current kernel sources; a summary stamped for other sources reads as stale."""
+    import json
+    b = _bench()
+    stamp = b.source_stamp()
+    assert len(stamp) == 16 and stamp == b.source_stamp()
+    good = {"source_stamp": stamp, "passes": {"ffts3d_512_f32_fwd1": {"bytes_per_launch": 123.0}}}
+    path = tmp_path / "traffic.json"
+    path.write_text(json.dumps(good))
+    monkeypatch.setattr(b, "TRAFFIC_PATH", str(path))
+    assert b._traffic_summary("ffts3d_512_f32_fwd1") == (123.0, stamp)
+    assert b._traffic_summary("ffts3d_512_f32_fwd9") == (None, stamp)
+    good["source_stamp"] = "0" * 16
+    path.write_text(json.dumps(good))
+    assert b._traffic_summary("ffts3d_512_f32_fwd1") == (None, "0" * 16)
