@@ -176,17 +176,23 @@ int fftm_mi_ffts_num_passes( fftm_mi_ffts h, int32_t *forward_passes, int32_t *b
  * passed; the pass reads and writes exactly what it does inside that call.  Used to
  * time individual kernels; it has no counterpart in the reference. */
 int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, const void *in, void *out, int32_t sync );
-/* Variant number (0..31) of one pass's kernel: a tile shape 0..7 plus 8 * the tile
- * order.  Orders: 0 one contiguous range of tiles per XCD, 1 the tiles dealt to the
+/* Variant number (0..95) of one pass's kernel: a tile shape 0..7 plus 8 * the tile
+ * order 0..3 plus 32 * the radix schedule 0..2.  Orders: 0 one contiguous range of tiles per XCD, 1 the tiles dealt to the
  * XCDs round-robin, 2 and 3 chunks of 8 and of 32 consecutive tiles dealt
  * round-robin.  Shapes:
  * 0 default, 1 half the columns per workgroup, 2 twice the threads per column, 3 the
  * tile of 4 exchanged through LDS one component at a time, 4 twice the columns
  * (256-byte row segments, up to 160 KB of LDS), 5 twice the columns and the threads
  * per column, 6 the tile of 0 exchanged through LDS one component at a time (half the
- * LDS per workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Results are bitwise
- * identical across variants; only the speed differs.  Set by the tuner or by hand.
- * FP16-storage passes (FFTM_MI_F32_H16) have shapes 0, 6 and 7.  Lengths compiled
+ * LDS per workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Radix
+ * schedules (strided FP32 C2C passes of length 512 and 1024, shapes 0 and 6): 0 the
+ * default factorisation (8 8 8, 16 8 8), 1 the alternative 8 16 4 / 8 16 8, 2 that
+ * alternative with its radix-16 stage as FP16 matrix-core products on hi + lo operand
+ * splits with FP32 accumulation.  Within one radix schedule results are bitwise
+ * identical across shapes and orders; another schedule rounds differently (FP32
+ * accuracy either way).  Set by the tuner or by hand.
+ * FP16-storage passes (FFTM_MI_F32_H16) have shapes 0, 6 and 7, hi + lo storage passes
+ * (FFTM_MI_F32_H16X2) shapes 0 and 6.  Lengths compiled
  * with the default shape only (below 64, and the 3/5/7-smooth lengths) run the default
  * shape whatever the variant; plane and any-radix passes also keep their own tile order. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
