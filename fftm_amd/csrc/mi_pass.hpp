@@ -205,6 +205,7 @@ struct pass_ctx
     const pass_params<T>   *p;
     const filter_params<T> *f;
     LW                     *lds;
+    const cpx<T>           *twl; // stage twiddles: the workgroup's LDS copy, or p->tw
     int                     c, t;
     int64_t                 tile;
     int64_t                 ia, ib, oa, ob;
@@ -729,6 +730,13 @@ struct pass_kernel
 
     using ctx = pass_ctx<T, LW>;
 
+    // The stage twiddles are read from an LDS copy of the table: a twiddle load through
+    // the vector memory path returns behind the tile loads issued before it, so under full
+    // HBM traffic every stage would wait on the memory queue.  LDS loads have their own
+    // counter.  Same values, same arithmetic.
+    static constexpr bool TWL = RP.n > 1 && N * (int)sizeof( C ) <= 8192 &&
+                                LDS * (int)sizeof( LW ) + N * (int)sizeof( C ) <= lds_limit_wide_bytes() - 8192;
+
     // ---- LDS exchange helpers: P = 2 moves whole values, P = 0 / 1 the real /
     // imaginary components (split exchange) ------------------------------------------
     template <int P>
@@ -1100,6 +1108,8 @@ struct pass_kernel
             // variants keep registers for occupancy instead and fetch them after it)
             constexpr bool TWPF = !SPLIT;
             C              tws[NB2][R2];
+            if constexpr ( TWL && S == 0 )
+                __syncthreads(); // the LDS twiddle table is complete
             if constexpr ( TWPF )
             {
 #pragma unroll
@@ -1111,7 +1121,7 @@ struct pass_kernel
                         const int k = j % NS2;
 #pragma unroll
                         for ( int r = 1; r < R2; ++r )
-                            tws[b][r] = x.p->tw[r * k * TST];
+                            tws[b][r] = x.twl[r * k * TST];
                     }
                 }
             }
@@ -1149,7 +1159,7 @@ struct pass_kernel
                         const int k = j % NS2;
 #pragma unroll
                         for ( int r = 1; r < R2; ++r )
-                            tws[b][r] = x.p->tw[r * k * TST];
+                            tws[b][r] = x.twl[r * k * TST];
                     }
 #pragma unroll
                     for ( int r = 1; r < R2; ++r )
@@ -1193,6 +1203,15 @@ struct pass_kernel
         x.f          = f;
         x.lds        = lds;
         const int tid = threadIdx.x;
+        if constexpr ( TWL )
+        {
+            __shared__ C twb[N];
+            for ( int i = tid; i < N; i += NT )
+                twb[i] = p.tw[i];
+            x.twl = twb;
+        }
+        else
+            x.twl = p.tw;
         x.c          = LOADB ? tid % TW : tid / TPC;
         x.t          = LOADB ? tid / TW : tid % TPC;
         x.tile       = ordered_tile( p.vec );
