@@ -206,6 +206,9 @@ struct pass_ctx
     const filter_params<T> *f;
     LW                     *lds;
     const cpx<T>           *twl; // stage twiddles: the workgroup's LDS copy, or p->tw
+    cpx<T>                 *twb; // the LDS copy, filled from twr at the first stage exchange
+    cpx<T>                  twr[4];
+    bool                    twpend;
     int                     c, t;
     int64_t                 tile;
     int64_t                 ia, ib, oa, ob;
@@ -736,6 +739,10 @@ struct pass_kernel
     // counter.  Same values, same arithmetic.
     static constexpr bool TWL = RP.n > 1 && N * (int)sizeof( C ) <= 8192 &&
                                 LDS * (int)sizeof( LW ) + N * (int)sizeof( C ) <= lds_limit_wide_bytes() - 8192;
+    // The table is loaded into registers before the tile (at most 4 values per thread) and
+    // written to LDS after the tile's loads are issued, so that no tile load waits for it.
+    static constexpr int  TWPT = ( N + NT - 1 ) / NT;
+    static constexpr bool TWR  = TWL && TWPT <= 4;
 
     // ---- LDS exchange helpers: P = 2 moves whole values, P = 0 / 1 the real /
     // imaginary components (split exchange) ------------------------------------------
@@ -1109,7 +1116,23 @@ struct pass_kernel
             constexpr bool TWPF = !SPLIT;
             C              tws[NB2][R2];
             if constexpr ( TWL && S == 0 )
+            {
+                if constexpr ( TWR )
+                {
+                    if ( x.twpend )
+                    {
+#pragma unroll
+                        for ( int i = 0; i < TWPT; ++i )
+                        {
+                            const int e = (int)threadIdx.x + i * NT;
+                            if ( N % NT == 0 || e < N )
+                                x.twb[e] = x.twr[i];
+                        }
+                        x.twpend = false;
+                    }
+                }
                 __syncthreads(); // the LDS twiddle table is complete
+            }
             if constexpr ( TWPF )
             {
 #pragma unroll
@@ -1203,12 +1226,27 @@ struct pass_kernel
         x.f          = f;
         x.lds        = lds;
         const int tid = threadIdx.x;
+        x.twpend = false;
         if constexpr ( TWL )
         {
             __shared__ C twb[N];
-            for ( int i = tid; i < N; i += NT )
-                twb[i] = p.tw[i];
             x.twl = twb;
+            x.twb = twb;
+            if constexpr ( TWR )
+            {
+#pragma unroll
+                for ( int i = 0; i < TWPT; ++i )
+                {
+                    const int e = tid + i * NT;
+                    x.twr[i]    = p.tw[N % NT == 0 || e < N ? e : 0];
+                }
+                x.twpend = true;
+            }
+            else
+            {
+                for ( int i = tid; i < N; i += NT )
+                    twb[i] = p.tw[i];
+            }
         }
         else
             x.twl = p.tw;
