@@ -73,6 +73,7 @@ struct pass_rt
     int        gen_cols = 1;
     int        wmode    = 0; // real-axis bin weights (adjoints), see pass_params::wmode
     int        var      = 0; // tile-shape variant of the fixed-length kernel
+    bool       rev      = false; // tiles in reverse order (alternate passes of a transform pair)
     double     scale    = 1.0;
     // FP16-stored spectra (mixed precision): IO_H16_IN / IO_H16_OUT bits, exponent
     // strides (input: batch dims and transform axis; output: batch dims) and the byte
@@ -346,7 +347,7 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     const int  shape = variant_shape_of( r.var );
     const int  rr    = variant_vec_order( r.var );
     const int  rs    = variant_rs_of( r.var );
-    p.vec |= rr;
+    p.vec |= rr | ( r.rev ? VEC_REV : 0 );
     if constexpr ( std::is_same<T, float>::value )
     {
         if ( r.iom )
@@ -1470,6 +1471,14 @@ int fftm_mi_ffts_create( int32_t layout, const int64_t *n, int32_t precision, in
         rc = prepare_passes( p->bwd, p->prec, &p->tb );
     if ( rc != FFTM_MI_SUCCESS )
         return rc;
+    // Every second pass of the forward + backward sequence runs its tiles last-first.  Each
+    // pass walks its columns in increasing memory order and writes its output in that
+    // order, so the reversed pass after it starts on the most recently written data, which
+    // the memory-side cache (256 MB on MI355X) still holds.  Results are unchanged.
+    for ( size_t i = 0; i < p->fwd.size(); ++i )
+        p->fwd[i].rev = ( i & 1 ) != 0;
+    for ( size_t i = 0; i < p->bwd.size(); ++i )
+        p->bwd[i].rev = ( ( p->fwd.size() + i ) & 1 ) != 0;
     p->work_size = ws;
     p->sg        = make_spec_geom( L );
     p->scratch   = static_cast<double *>( scratch_for_current_device( &p->scratch_err ) );

@@ -358,19 +358,24 @@ constexpr int NUM_ORDERS  = 4;
 constexpr int VEC_RR      = 4;
 constexpr int VEC_CHUNK8  = 8;
 constexpr int VEC_CHUNK32 = 16;
+// bit 5 of pass_params::vec: run the tiles in reverse order (last tile first), so that a
+// pass reads first what the pass before it wrote last, while that is still in the
+// memory-side cache
+constexpr int VEC_REV     = 32;
 constexpr int variant_shape_of( int var ) { return var & 7; }
 constexpr int variant_order_of( int var ) { return ( var >> 3 ) & 3; }
 constexpr int variant_rs_of( int var ) { return var >> 5; }
 
 __device__ __forceinline__ int64_t ordered_tile( int order_bits )
 {
+    const int64_t wg = ( order_bits & VEC_REV ) ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
     if ( order_bits & VEC_RR )
-        return (int64_t)blockIdx.x;
+        return wg;
     if ( order_bits & VEC_CHUNK8 )
-        return xcd_chunk( blockIdx.x, gridDim.x, 8 );
+        return xcd_chunk( wg, gridDim.x, 8 );
     if ( order_bits & VEC_CHUNK32 )
-        return xcd_chunk( blockIdx.x, gridDim.x, 32 );
-    return xcd_tile( blockIdx.x, gridDim.x );
+        return xcd_chunk( wg, gridDim.x, 32 );
+    return xcd_tile( wg, gridDim.x );
 }
 
 constexpr bool variant_split( int var ) { return var >= 6 || var == 3; }
