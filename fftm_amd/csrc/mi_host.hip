@@ -835,7 +835,7 @@ int plane_partner( int32_t layout )
 }
 
 // Replace the real-axis pass and the C2C pass along `B` by one plane pass when the
-// (B x A) half-spectrum plane fits in LDS and there are enough planes to fill the chip.
+// (B x A) half-spectrum plane fits in LDS and there are at least 64 planes.
 void fuse_plane( const ffts_layout &L, int32_t layout, int prec, std::vector<pass_rt> &fwd, std::vector<pass_rt> &bwd )
 {
     const int     d  = L.dim;
@@ -852,7 +852,10 @@ void fuse_plane( const ffts_layout &L, int32_t layout, int prec, std::vector<pas
             nplanes *= L.n[k];
             other[no++] = k;
         }
-    if ( nplanes < 512 )
+    // Fewer planes than CUs still pay off: the fused pair has four passes instead of six,
+    // and problems this small are bound by their launches (128^3 FP64 round trip 62 ->
+    // 50 us, 64^3 30 -> 24 us).  Below 64 planes the split passes keep more workgroups.
+    if ( nplanes < 64 )
         return;
     pass_rt p;
     p.plane   = true;
