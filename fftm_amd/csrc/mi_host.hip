@@ -306,6 +306,19 @@ bool even_line_offsets( const pass_rt &r, int64_t s0, int64_t s1, int64_t s2 )
     return ( r.d0 <= 1 || s0 % 2 == 0 ) && ( r.d1 <= 1 || s1 % 2 == 0 ) && ( d2 <= 1 || s2 % 2 == 0 );
 }
 
+// Multiply-and-shift constants for dividing any n < 2^31 by d (1 <= d < 2^31): with
+// l = ceil(log2 d) and m = ceil(2^(31 + l) / d), n / d = (n m) >> (31 + l) exactly
+// (Granlund and Montgomery, "Division by invariant integers using multiplication", 1994,
+// theorem 4.2: 2^(31 + l) <= m d <= 2^(31 + l) + 2^l).  m <= 2^32, so n m < 2^63.
+void div_magic( int64_t d, uint64_t &m, int32_t &sh )
+{
+    int l = 0;
+    while ( ( (int64_t)1 << l ) < d )
+        ++l;
+    sh = 31 + l;
+    m  = ( ( (uint64_t)1 << sh ) + (uint64_t)d - 1 ) / (uint64_t)d;
+}
+
 int32_t pair_access_bits( const pass_rt &r, const void *src, const void *dst )
 {
     int32_t bits = 0;
@@ -339,6 +352,11 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.os0     = r.os0;
     p.os1     = r.os1;
     p.os2     = r.os2;
+    if ( r.ncols < ( (int64_t)1 << 31 ) && r.d0 >= 1 && r.d1 >= 1 )
+    {
+        div_magic( r.d0, p.mg0, p.sh0 );
+        div_magic( r.d1, p.mg1, p.sh1 );
+    }
     p.store_b = r.storeb ? 1 : 0;
     p.wmode   = r.wmode;
     p.vec     = pair_access_bits( r, src, dst );

@@ -65,7 +65,17 @@ struct pass_params
     int8_t       *xout;                 // exponents of the FP16 output lines (written here)
     int64_t       xs0, xs1, xs2, xsn;   // input exponent strides: batch dims, transform axis
     int64_t       ys0, ys1, ys2;        // output exponent strides: batch dims
+    // division by d0 and d1 as a multiply and a shift (q = col * mg >> sh, exact for every
+    // col < 2^31; mg = 0: divide)
+    uint64_t      mg0, mg1;
+    int32_t       sh0, sh1;
 };
+
+// n / d with the pass's multiply-and-shift constants (n < 2^31 whenever m != 0)
+__device__ __forceinline__ int64_t div_col( int64_t n, int64_t d, uint64_t m, int sh )
+{
+    return m ? (int64_t)( ( (uint64_t)n * m ) >> sh ) : n / d;
+}
 
 typedef _Float16 h16x2 __attribute__( ( ext_vector_type( 2 ) ) );
 typedef _Float16 h16x4 __attribute__( ( ext_vector_type( 4 ) ) );
@@ -449,10 +459,10 @@ constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 template <typename T>
 __device__ __forceinline__ void col_offsets( const pass_params<T> &p, int64_t col, int64_t &io, int64_t &oo )
 {
-    const int64_t c0 = col % p.d0;
-    const int64_t r  = col / p.d0;
-    const int64_t c1 = r % p.d1;
-    const int64_t c2 = r / p.d1;
+    const int64_t r  = div_col( col, p.d0, p.mg0, p.sh0 );
+    const int64_t c0 = col - r * p.d0;
+    const int64_t c2 = div_col( r, p.d1, p.mg1, p.sh1 );
+    const int64_t c1 = r - c2 * p.d1;
     io               = c0 * p.is0 + c1 * p.is1 + c2 * p.is2;
     oo               = c0 * p.os0 + c1 * p.os1 + c2 * p.os2;
 }
@@ -461,10 +471,10 @@ __device__ __forceinline__ void col_offsets( const pass_params<T> &p, int64_t co
 template <typename T>
 __device__ __forceinline__ void col_exponents( const pass_params<T> &p, int64_t col, int64_t &xi, int64_t &yo )
 {
-    const int64_t c0 = col % p.d0;
-    const int64_t r  = col / p.d0;
-    const int64_t c1 = r % p.d1;
-    const int64_t c2 = r / p.d1;
+    const int64_t r  = div_col( col, p.d0, p.mg0, p.sh0 );
+    const int64_t c0 = col - r * p.d0;
+    const int64_t c2 = div_col( r, p.d1, p.mg1, p.sh1 );
+    const int64_t c1 = r - c2 * p.d1;
     xi               = c0 * p.xs0 + c1 * p.xs1 + c2 * p.xs2;
     yo               = c0 * p.ys0 + c1 * p.ys1 + c2 * p.ys2;
 }
