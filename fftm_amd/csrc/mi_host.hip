@@ -1121,6 +1121,9 @@ spec_geom make_spec_geom( const ffts_layout &L )
     }
     g.kmax2   = kmax2 > 0 ? kmax2 : 1; // all-ones sizes: every bin has k = 0
     g.nshells = 1;
+    if ( g.total < ( (int64_t)1 << 31 ) )
+        for ( int m = 0; m < L.dim; ++m )
+            div_magic( g.ext[m], g.mg[m], g.sh[m] );
     return g;
 }
 
@@ -1768,6 +1771,8 @@ filter_params<T> make_filter_params( const fftm_mi_ffts_s &h, const fftm_mi_spec
     {
         const int a = h.sg.ax[m];
         f.ext[m]    = h.sg.ext[m];
+        f.mg[m]     = h.sg.mg[m];
+        f.sh[m]     = h.sg.sh[m];
         f.nlen[m]   = a == h.sg.half_axis ? 0 : h.sg.n[a];
         if ( ( op.kind == FFTM_MI_OP_DERIVATIVE || op.kind == FFTM_MI_OP_POISSON_GRADIENT ) && a == op.axis )
             f.dax = m;

@@ -135,6 +135,8 @@ struct filter_params
     int64_t  kmax2;    // sum over axes of (n_a / 2)^2 (radial shells)
     double   klo2, khi2; // OP_BANDPASS: squared band edges
     const T *coeffs;   // OP_SHELL coefficients (T[nshells])
+    uint64_t mg[4];    // division by ext[m] as a multiply and a shift (mg = 0: divide)
+    int32_t  sh[4];
 };
 
 // Radial shell of a bin with |k|^2 = k2: the largest s with s^2 kmax2 <= S^2 k2, i.e.
@@ -165,8 +167,9 @@ __device__ __forceinline__ float filter_k2( const filter_params<T> &f, int64_t o
     kd       = 0.f;
     for ( int m = f.nd - 1; m >= 0; --m )
     {
-        const int64_t idx = off % f.ext[m];
-        off /= f.ext[m];
+        const int64_t q   = div_col( off, f.ext[m], f.mg[m], f.sh[m] );
+        const int64_t idx = off - q * f.ext[m];
+        off               = q;
         const int64_t n = f.nlen[m];
         const int64_t k = ( n == 0 || idx <= n / 2 ) ? idx : idx - n;
         k2 += (float)( k * k );

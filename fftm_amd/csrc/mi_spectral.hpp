@@ -25,6 +25,8 @@ struct spec_geom
     int64_t total;     // number of complex bins
     int64_t kmax2;     // sum over axes of (n_a / 2)^2
     int32_t nshells;
+    uint64_t mg[4];    // division by ext[m] as a multiply and a shift (mg = 0: divide)
+    int32_t  sh[4];
 };
 
 __device__ __forceinline__ int spec_shell( const spec_geom &g, int64_t i )
@@ -33,8 +35,9 @@ __device__ __forceinline__ int spec_shell( const spec_geom &g, int64_t i )
     int64_t r  = i;
     for ( int m = g.nd - 1; m >= 0; --m )
     {
-        const int64_t idx = r % g.ext[m];
-        r /= g.ext[m];
+        const int64_t q   = div_col( r, g.ext[m], g.mg[m], g.sh[m] );
+        const int64_t idx = r - q * g.ext[m];
+        r                 = q;
         const int     a   = g.ax[m];
         const int64_t na  = g.n[a];
         const int64_t k   = ( a == g.half_axis || idx <= na / 2 ) ? idx : idx - na;
