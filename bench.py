@@ -84,11 +84,13 @@ def _traffic_summary(kernel_key):
     return float(v["bytes_per_launch"]), stamp
 
 
-def write_traffic(trace_csv, fetch_csv, write_csv, passes=6, pairs=30):
-    """profiles/traffic.json from the rocprofv3 runs of bench.py --no-cpu --no-check --no-4d
-    --no-extra --calibrate-copy --tuning-cache C (trace, --pmc FETCH_SIZE, --pmc WRITE_SIZE):
-    the last pairs * passes k_pass dispatches are the timed loops, pass = position mod 6;
-    the counters are scaled by their reading for the 1 GiB copy (read 2^30 B, written 2^30 B)."""
+def write_traffic(trace_csv, fetch_csv, write_csv, passes=6, pairs=30, size=512):
+    """profiles/traffic.json from the rocprofv3 runs of bench.py --size S --no-cpu --no-check
+    --no-4d --no-extra --calibrate-copy --tuning-cache C (trace, --pmc FETCH_SIZE, --pmc
+    WRITE_SIZE): the last pairs * passes k_pass dispatches are the timed loops, pass =
+    position mod 6; the counters are scaled by their reading for the 1 GiB copy (read 2^30 B,
+    written 2^30 B).  Entries are keyed by size; those of other sizes already in the file
+    are kept when they were measured on the same kernel sources."""
     import csv
 
     def rows(path, counter=None):
@@ -115,8 +117,16 @@ def write_traffic(trace_csv, fetch_csv, write_csv, passes=6, pairs=30):
     fk = [v * 1024.0 * fr for name, v in fetch if "k_pass" in name][-pairs * passes:]
     wk = [v * 1024.0 * fw for name, v in write if "k_pass" in name][-pairs * passes:]
     tr = [r for r in rows(trace_csv) if "k_pass" in r["Kernel_Name"]][-pairs * passes:]
-    names = [f"ffts3d_512_f32_{'bwd' if k >= 3 else 'fwd'}{k % 3}" for k in range(passes)]
-    out = {"source_stamp": source_stamp(), "fetch_scale": fr, "write_scale": fw, "passes": {}}
+    names = [f"ffts3d_{size}_f32_{'bwd' if k >= 3 else 'fwd'}{k % 3}" for k in range(passes)]
+    stamp = source_stamp()
+    out = {"source_stamp": stamp, "fetch_scale": fr, "write_scale": fw, "passes": {}}
+    try:
+        with open(TRAFFIC_PATH) as f:
+            old = json.load(f)
+        if old.get("source_stamp") == stamp:
+            out["passes"].update({k: v for k, v in old.get("passes", {}).items() if k not in names})
+    except (OSError, ValueError):
+        pass
     for k, name in enumerate(names):
         rd = [fk[i] for i in range(k, len(fk), passes)]
         wr = [wk[i] for i in range(k, len(wk), passes)]
@@ -692,7 +702,7 @@ def main():
                          "of a run with --no-cpu --no-check --no-4d --no-extra --calibrate-copy, then exit")
     args = ap.parse_args()
     if args.write_traffic:
-        print(json.dumps(write_traffic(*args.write_traffic, pairs=3 * args.steps), indent=1))
+        print(json.dumps(write_traffic(*args.write_traffic, pairs=3 * args.steps, size=args.size), indent=1))
         return
     global TUNING_CACHE
     TUNING_CACHE = args.tuning_cache

@@ -73,3 +73,50 @@ def test_traffic_figure_is_stamped_with_the_kernel_sources(tmp_path, monkeypatch
     good["source_stamp"] = "0" * 16
     path.write_text(json.dumps(good))
     assert b._traffic_summary("ffts3d_512_f32_fwd1") == (None, "0" * 16)
+
+
+def _traffic_csvs(tmp_path, tag, npass=6, pairs=2, read_kib=100.0, write_kib=50.0):
+    """Synthetic rocprofv3 csv files: a 1 GiB copy (read 512 MiB, written 1 GiB as the
+    counters report it), then pairs * npass k_pass dispatches."""
+    trace = tmp_path / f"{tag}_trace.csv"
+    fetch = tmp_path / f"{tag}_fetch.csv"
+    write = tmp_path / f"{tag}_write.csv"
+    t_rows = ["Kernel_Name,Start_Timestamp,End_Timestamp"]
+    f_rows = ["Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value"]
+    w_rows = ["Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value"]
+    f_rows.append(f"0,__amd_rocclr_copyBuffer,FETCH_SIZE,{512 * 1024}")
+    w_rows.append(f"0,__amd_rocclr_copyBuffer,WRITE_SIZE,{1024 * 1024}")
+    for d in range(pairs * npass):
+        k = d % npass
+        t_rows.append(f"k_pass<{k}>,{1000 * d},{1000 * d + 200 + k}")
+        f_rows.append(f"{d + 1},k_pass<{k}>,FETCH_SIZE,{read_kib * (k + 1)}")
+        w_rows.append(f"{d + 1},k_pass<{k}>,WRITE_SIZE,{write_kib * (k + 1)}")
+    trace.write_text("\n".join(t_rows) + "\n")
+    fetch.write_text("\n".join(f_rows) + "\n")
+    write.write_text("\n".join(w_rows) + "\n")
+    return str(trace), str(fetch), str(write)
+
+
+def test_traffic_entries_are_keyed_by_size_and_merged_per_build(tmp_path, monkeypatch):
+    """write_traffic keys the passes by transform size: a 1024^3 run adds its entries next
+    to the 512^3 ones measured on the same kernel sources instead of replacing them, and
+    the counters are scaled by the copy calibration (FETCH x 2, WRITE x 1)."""
+    import json
+    b = _bench()
+    path = tmp_path / "traffic.json"
+    monkeypatch.setattr(b, "TRAFFIC_PATH", str(path))
+    b.write_traffic(*_traffic_csvs(tmp_path, "a"), pairs=2, size=512)
+    b.write_traffic(*_traffic_csvs(tmp_path, "b", read_kib=200.0), pairs=2, size=1024)
+    data = json.loads(path.read_text())
+    assert data["source_stamp"] == b.source_stamp()
+    p512, p1024 = data["passes"]["ffts3d_512_f32_fwd1"], data["passes"]["ffts3d_1024_f32_fwd1"]
+    assert p512["read"] == 2 * 100.0 * 2 * 1024 and p512["write"] == 50.0 * 2 * 1024
+    assert p1024["read"] == 2 * 200.0 * 2 * 1024
+    assert p512["avg_us"] == 0.201
+    assert b._traffic_summary("ffts3d_512_f32_fwd1")[0] == p512["bytes_per_launch"]
+    # entries measured on other kernel sources are dropped, not merged
+    data["source_stamp"] = "0" * 16
+    path.write_text(json.dumps(data))
+    b.write_traffic(*_traffic_csvs(tmp_path, "c"), pairs=2, size=1024)
+    assert sorted(json.loads(path.read_text())["passes"]) == [f"ffts3d_1024_f32_{d}{k}" for d in ("bwd", "fwd")
+                                                                for k in range(3)]
