@@ -352,7 +352,9 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
     p.os0     = r.os0;
     p.os1     = r.os1;
     p.os2     = r.os2;
-    if ( r.ncols < ( (int64_t)1 << 31 ) && r.d0 >= 1 && r.d1 >= 1 )
+    // every column index a kernel forms stays below ncols plus one tile (at most 2048
+    // columns), inside the constants' exact range n < 2^31
+    if ( r.ncols < ( (int64_t)1 << 31 ) - 4096 && r.d0 >= 1 && r.d1 >= 1 )
     {
         div_magic( r.d0, p.mg0, p.sh0 );
         div_magic( r.d1, p.mg1, p.sh1 );
