@@ -442,7 +442,7 @@ int default_order( const pass_rt &p )
     return big ? 3 : 0;
 }
 
-// a variant number this pass can run: FP16-storage passes have tile shapes 0, 6, 7; the
+// a variant number this pass can run: FP16-storage passes have tile shapes 0, 2, 6, 7; the
 // alternative radix schedules exist for the strided FP32-arithmetic C2C passes of lengths
 // 512 and 1024 in tile shapes 0 and 6
 bool pass_variant_ok( const pass_rt &r, int var, int prec )
@@ -452,7 +452,7 @@ bool pass_variant_ok( const pass_rt &r, int var, int prec )
     if ( r.iom && !mixed_variant_ok( var ) )
         return false;
     const int rs = variant_rs_of( var );
-    if ( ( r.iom & IO_SPLIT ) && ( variant_shape_of( var ) == 7 || rs != 0 ) )
+    if ( ( r.iom & IO_SPLIT ) && ( ( variant_shape_of( var ) != 0 && variant_shape_of( var ) != 6 ) || rs != 0 ) )
         return false; // hi + lo storage: tile shapes 0 and 6, default radix schedule
     return rs == 0 ||
            ( !r.plane && !r.generic && !r.filter && rs_supported( r.N, r.kind, r.loadb, prec, variant_shape_of( var ) ) );

@@ -196,6 +196,7 @@ hipError_t launch_mixed_n( const pass_params<float> &p, int kind, bool inv, bool
     switch ( var )
     {
     case 0: return launch_mixed_v<N, 0>( p, kind, inv, loadb, iom, s );
+    case 2: return launch_mixed_v<N, 2>( p, kind, inv, loadb, iom, s );
     case 6: return launch_mixed_v<N, 6>( p, kind, inv, loadb, iom, s );
     case 7: return launch_mixed_v<N, 7>( p, kind, inv, loadb, iom, s );
     default: return hipErrorInvalidValue;

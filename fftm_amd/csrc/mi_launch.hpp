@@ -49,7 +49,7 @@ hipError_t launch_plane_rx_f32( const plane_params<float> &p, int n, bool fwd, h
 hipError_t launch_plane_rx_f64( const plane_params<double> &p, int n, bool fwd, hipStream_t s );
 
 // FP16-stored spectra with FP32 arithmetic (iom: bit 0 FP16 input, bit 1 FP16 output);
-// lengths 128..2048, tile-shape variants 0, 6, 7
+// lengths 128..2048, tile-shape variants 0, 2, 6, 7
 inline bool mixed_supported( int64_t N )
 {
     return N >= 128 && N <= 2048 && ( N & ( N - 1 ) ) == 0;
@@ -57,7 +57,7 @@ inline bool mixed_supported( int64_t N )
 inline bool mixed_variant_ok( int var )
 {
     const int v = variant_shape_of( var );
-    return v == 0 || v == 6 || v == 7;
+    return v == 0 || v == 2 || v == 6 || v == 7;
 }
 // a variant number: a tile shape 0..NUM_VARIANTS-1, plus 8 * the tile order (0..3), plus 32 *
 // the radix schedule (0 default; 1 the alternative schedule; 2 the alternative schedule with
