@@ -452,8 +452,8 @@ bool pass_variant_ok( const pass_rt &r, int var, int prec )
     if ( r.iom && !mixed_variant_ok( var ) )
         return false;
     const int rs = variant_rs_of( var );
-    if ( ( r.iom & IO_SPLIT ) && ( ( variant_shape_of( var ) != 0 && variant_shape_of( var ) != 6 ) || rs != 0 ) )
-        return false; // hi + lo storage: tile shapes 0 and 6, default radix schedule
+    if ( ( r.iom & IO_SPLIT ) && ( variant_shape_of( var ) == 7 || rs != 0 ) )
+        return false; // hi + lo storage: tile shapes 0, 2 and 6, default radix schedule
     return rs == 0 ||
            ( !r.plane && !r.generic && !r.filter && rs_supported( r.N, r.kind, r.loadb, prec, variant_shape_of( var ) ) );
 }

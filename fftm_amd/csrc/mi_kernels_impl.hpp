@@ -167,7 +167,7 @@ hipError_t launch_mixed_v( const pass_params<float> &p, int kind, bool inv, bool
 // tile-shape variants compiled for the FP16-storage kernels
 constexpr bool mixed_variant( int var ) { return var == 0 || var == 6 || var == 7; }
 
-// hi + lo FP16-stored spectra (IO_SPLIT): tile shapes 0 and 6
+// hi + lo FP16-stored spectra (IO_SPLIT): tile shapes 0, 2 and 6
 template <int N>
 hipError_t launch_mixed_split_n( const pass_params<float> &p, int kind, bool inv, bool loadb, int var, int iom,
                                  hipStream_t s )
@@ -185,6 +185,8 @@ hipError_t launch_mixed_split_n( const pass_params<float> &p, int kind, bool inv
     };
     if ( var == 0 )
         return go( std::integral_constant<int, 0>{} );
+    if ( var == 2 )
+        return go( std::integral_constant<int, 2>{} );
     if ( var == 6 )
         return go( std::integral_constant<int, 6>{} );
     return hipErrorInvalidValue;

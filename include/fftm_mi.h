@@ -192,7 +192,7 @@ int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, cons
  * identical across shapes and orders; another schedule rounds differently (FP32
  * accuracy either way).  Set by the tuner or by hand.
  * FP16-storage passes (FFTM_MI_F32_H16) have shapes 0, 2, 6 and 7, hi + lo storage passes
- * (FFTM_MI_F32_H16X2) shapes 0 and 6.  Lengths compiled
+ * (FFTM_MI_F32_H16X2) shapes 0, 2 and 6.  Lengths compiled
  * with the default shape only (below 64, and the 3/5/7-smooth lengths) run the default
  * shape whatever the variant; plane and any-radix passes also keep their own tile order. */
 int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t index, int32_t variant );
