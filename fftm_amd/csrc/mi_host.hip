@@ -455,7 +455,7 @@ bool pass_variant_ok( const pass_rt &r, int var, int prec )
     if ( ( r.iom & IO_SPLIT ) && ( variant_shape_of( var ) == 7 || rs != 0 ) )
         return false; // hi + lo storage: tile shapes 0, 2 and 6, default radix schedule
     return rs == 0 ||
-           ( !r.plane && !r.generic && !r.filter && rs_supported( r.N, r.kind, r.loadb, prec, variant_shape_of( var ) ) );
+           ( !r.plane && !r.generic && !r.filter && rs_supported( r.N, r.kind, r.loadb, prec, variant_shape_of( var ), rs ) );
 }
 
 int prepare_passes( std::vector<pass_rt> &passes, int prec, tables *tb )

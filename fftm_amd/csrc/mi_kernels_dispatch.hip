@@ -149,9 +149,10 @@ hipError_t launch_pass_mixed_f32( const pass_params<float> &p, int N, int kind, 
 hipError_t launch_rs_f32_n512( const pass_params<float> &, bool, int, int, int, hipStream_t );
 hipError_t launch_rs_f32_n1024( const pass_params<float> &, bool, int, int, int, hipStream_t );
 
-bool rs_supported( int64_t N, int kind, bool loadb, int prec, int shape )
+bool rs_supported( int64_t N, int kind, bool loadb, int prec, int shape, int rs )
 {
-    return ( N == 512 || N == 1024 ) && kind == KIND_C2C && loadb && prec == 0 && ( shape == 0 || shape == 6 );
+    return ( N == 512 || N == 1024 ) && kind == KIND_C2C && loadb && prec == 0 &&
+           ( shape == 0 || shape == 6 || ( shape == 2 && rs == 1 ) );
 }
 
 hipError_t launch_pass_rs_f32( const pass_params<float> &p, int N, bool inv, int shape, int rs, int iom, hipStream_t s )

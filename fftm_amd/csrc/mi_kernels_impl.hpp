@@ -263,6 +263,8 @@ hipError_t launch_rs_n( const pass_params<float> &p, bool inv, int shape, int rs
         return rs == 1 ? launch_rs_v<N, 0, 1>( p, inv, iom, s ) : launch_rs_v<N, 0, 2>( p, inv, iom, s );
     if ( shape == 6 )
         return rs == 1 ? launch_rs_v<N, 6, 1>( p, inv, iom, s ) : launch_rs_v<N, 6, 2>( p, inv, iom, s );
+    if ( shape == 2 && rs == 1 )
+        return launch_rs_v<N, 2, 1>( p, inv, iom, s );
     return hipErrorInvalidValue;
 }
 

@@ -67,8 +67,8 @@ inline bool variant_number_ok( int var )
     return var >= 0 && variant_rs_of( var ) <= 2;
 }
 // alternative radix schedules are compiled for strided C2C passes of length 512 and 1024
-// with FP32 arithmetic (prec 0), tile shapes 0 and 6
-bool       rs_supported( int64_t N, int kind, bool loadb, int prec, int shape );
+// with FP32 arithmetic (prec 0), tile shapes 0 and 6 (and 2 with schedule 1)
+bool       rs_supported( int64_t N, int kind, bool loadb, int prec, int shape, int rs );
 hipError_t launch_pass_rs_f32( const pass_params<float> &p, int N, bool inv, int shape, int rs, int iom, hipStream_t s );
 // the order bits of pass_params::vec for a variant number
 inline int variant_vec_order( int var )

@@ -35,8 +35,7 @@ def _make(dims, storage="fp32"):
 
 
 @pytest.mark.parametrize("dims", [(512, 512, 64), (1024, 1024, 16)])
-@pytest.mark.parametrize("rs", [1, 2])
-@pytest.mark.parametrize("shape", [0, 6])
+@pytest.mark.parametrize("rs,shape", [(1, 0), (1, 6), (1, 2), (2, 0), (2, 6)])
 def test_radix_schedules_fp32_vs_oracle(gpu, dims, rs, shape):
     import torch
     print(f"case radix fp32 dims={dims} rs={rs} shape={shape}", flush=True)
@@ -103,7 +102,7 @@ def test_radix_candidates_in_the_tuning_cache(gpu, tmp_path):
             k, v = ln.split("=", 1)
             rec[k] = v
     # every strided C2C pass recorded candidates of both alternative schedules
-    for var in (32, 38, 64, 70):
+    for var in (32, 34, 38, 64, 70):
         keys = [k for k in rec if k.endswith(f"_CANDIDATE_{var:03d}_VALID")]
         assert len(keys) == 4, (var, keys)
         assert all(rec[k] == "1" for k in keys), (var, [rec[k] for k in keys])
@@ -123,7 +122,9 @@ def test_radix_schedules_rejected_where_not_compiled(gpu):
     h = f._handle
     # the z-R2C pass (real axis) and the 64-point passes have no alternative schedules
     assert L.fftm_mi_ffts_set_pass_variant(h, 0, 0, 32) == _lib.NOT_SUPPORTED
-    assert L.fftm_mi_ffts_set_pass_variant(h, 0, 1, 32 + 2) == _lib.NOT_SUPPORTED   # shape 2
+    assert L.fftm_mi_ffts_set_pass_variant(h, 0, 1, 32 + 1) == _lib.NOT_SUPPORTED   # shape 1
+    assert L.fftm_mi_ffts_set_pass_variant(h, 0, 1, 64 + 2) == _lib.NOT_SUPPORTED   # shape 2, matrix cores
+    assert L.fftm_mi_ffts_set_pass_variant(h, 0, 1, 32 + 2) == 0                    # shape 2, schedule 1
     assert L.fftm_mi_ffts_set_pass_variant(h, 0, 1, 96) == _lib.INVALID_VALUE       # no schedule 3
     assert L.fftm_mi_ffts_set_pass_variant(h, 0, 1, 64 + 8 * 3 + 6) == 0
     f.close()
