@@ -185,7 +185,8 @@ int fftm_mi_ffts_run_pass( fftm_mi_ffts h, int32_t backward, int32_t index, cons
  * (256-byte row segments, up to 160 KB of LDS), 5 twice the columns and the threads
  * per column, 6 the tile of 0 exchanged through LDS one component at a time (half the
  * LDS per workgroup), 7 as 6 with registers capped for 4 waves per SIMD.  Radix
- * schedules (strided FP32 C2C passes of length 512 and 1024, shapes 0 and 6): 0 the
+ * schedules (strided FP32 C2C passes of length 512 and 1024, shapes 0 and 6, and shape 2
+ * for schedule 1): 0 the
  * default factorisation (8 8 8, 16 8 8), 1 the alternative 8 16 4 / 8 16 8, 2 that
  * alternative with its radix-16 stage as FP16 matrix-core products on hi + lo operand
  * splits with FP32 accumulation.  Within one radix schedule results are bitwise
