@@ -359,6 +359,8 @@ hipError_t launch_one( const pass_rt &r, const tables &tb, const void *src, void
         div_magic( r.d0, p.mg0, p.sh0 );
         div_magic( r.d1, p.mg1, p.sh1 );
     }
+    else if ( r.N >= 32 )
+        return hipErrorInvalidValue; // those kernels compile no division fallback
     p.store_b = r.storeb ? 1 : 0;
     p.wmode   = r.wmode;
     p.vec     = pair_access_bits( r, src, dst );
@@ -444,7 +446,7 @@ int default_order( const pass_rt &p )
 
 // a variant number this pass can run: FP16-storage passes have tile shapes 0, 2, 6, 7; the
 // alternative radix schedules exist for the strided FP32-arithmetic C2C passes of lengths
-// 512 and 1024 in tile shapes 0 and 6
+// 512 and 1024 in tile shapes 0 and 6 (schedule 1 also in shape 2)
 bool pass_variant_ok( const pass_rt &r, int var, int prec )
 {
     if ( !variant_number_ok( var ) )
@@ -1917,11 +1919,12 @@ int fftm_mi_ffts_set_pass_variant( fftm_mi_ffts h, int32_t backward, int32_t ind
     if ( !variant_number_ok( variant ) )
         return fail( FFTM_MI_INVALID_VALUE, "fftm_mi_ffts_set_pass_variant: variant out of range" );
     if ( v[(size_t)index].iom && !mixed_variant_ok( variant ) )
-        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_set_pass_variant: FP16-storage passes have tile shapes 0, 6, 7" );
+        return fail( FFTM_MI_NOT_SUPPORTED, "fftm_mi_ffts_set_pass_variant: FP16-storage passes have tile shapes 0, 2, 6, 7" );
     if ( !pass_variant_ok( v[(size_t)index], variant, h->prec ) )
         return fail( FFTM_MI_NOT_SUPPORTED,
                      "fftm_mi_ffts_set_pass_variant: radix schedules 1 and 2 exist for strided C2C passes of length 512 "
-                     "and 1024 in tile shapes 0 and 6 (FP32 arithmetic)" );
+                     "and 1024 in tile shapes 0 and 6, schedule 1 also in shape 2 (FP32 arithmetic; hi + lo FP16 "
+                     "storage: tile shapes 0, 2 and 6, schedule 0)" );
     v[(size_t)index].var = variant;
     h->flt_ready         = false;
     return ok();

@@ -234,7 +234,7 @@ hipError_t launch_generic_t( const pass_params<T> &p, const radix_list &rl, int 
 
 // Alternative radix schedules (RS 1: make_rplan_alt on the vector ALUs, RS 2: its radix-16
 // stage on the matrix cores) of the strided C2C passes, FP32 arithmetic with FP32 or
-// FP16-stored lines, tile shapes 0 and 6.
+// FP16-stored lines, tile shapes 0 and 6 (schedule 1 also in shape 2).
 template <int N, int VAR, int RS>
 hipError_t launch_rs_v( const pass_params<float> &p, bool inv, int iom, hipStream_t s )
 {

@@ -71,10 +71,17 @@ struct pass_params
     int32_t       sh0, sh1;
 };
 
-// n / d with the pass's multiply-and-shift constants (n < 2^31 whenever m != 0)
+// n / d with the pass's multiply-and-shift constants (n < 2^31 whenever m != 0).  MG: the
+// launch always carries the constants, so no 64-bit division is compiled (pass kernels of
+// 32 or more points: their launches with more columns than the constants cover are refused
+// on the host, and such an array would exceed the device memory anyway)
+template <bool MG = false>
 __device__ __forceinline__ int64_t div_col( int64_t n, int64_t d, uint64_t m, int sh )
 {
-    return m ? (int64_t)( ( (uint64_t)n * m ) >> sh ) : n / d;
+    if constexpr ( MG )
+        return (int64_t)( ( (uint64_t)n * m ) >> sh );
+    else
+        return m ? (int64_t)( ( (uint64_t)n * m ) >> sh ) : n / d;
 }
 
 typedef _Float16 h16x2 __attribute__( ( ext_vector_type( 2 ) ) );
@@ -459,24 +466,24 @@ constexpr geom pick_geom( int N, int csize, bool loadb, int var = 0 )
 // ---------------------------------------------------------------------------
 // column address helpers
 // ---------------------------------------------------------------------------
-template <typename T>
+template <bool MG = false, typename T>
 __device__ __forceinline__ void col_offsets( const pass_params<T> &p, int64_t col, int64_t &io, int64_t &oo )
 {
-    const int64_t r  = div_col( col, p.d0, p.mg0, p.sh0 );
+    const int64_t r  = div_col<MG>( col, p.d0, p.mg0, p.sh0 );
     const int64_t c0 = col - r * p.d0;
-    const int64_t c2 = div_col( r, p.d1, p.mg1, p.sh1 );
+    const int64_t c2 = div_col<MG>( r, p.d1, p.mg1, p.sh1 );
     const int64_t c1 = r - c2 * p.d1;
     io               = c0 * p.is0 + c1 * p.is1 + c2 * p.is2;
     oo               = c0 * p.os0 + c1 * p.os1 + c2 * p.os2;
 }
 
 // exponent indices of column col: input lines (consumer side), output line (producer side)
-template <typename T>
+template <bool MG = false, typename T>
 __device__ __forceinline__ void col_exponents( const pass_params<T> &p, int64_t col, int64_t &xi, int64_t &yo )
 {
-    const int64_t r  = div_col( col, p.d0, p.mg0, p.sh0 );
+    const int64_t r  = div_col<MG>( col, p.d0, p.mg0, p.sh0 );
     const int64_t c0 = col - r * p.d0;
-    const int64_t c2 = div_col( r, p.d1, p.mg1, p.sh1 );
+    const int64_t c2 = div_col<MG>( r, p.d1, p.mg1, p.sh1 );
     const int64_t c1 = r - c2 * p.d1;
     xi               = c0 * p.xs0 + c1 * p.xs1 + c2 * p.xs2;
     yo               = c0 * p.ys0 + c1 * p.ys1 + c2 * p.ys2;
@@ -679,6 +686,8 @@ struct pass_kernel
     static constexpr int   PB  = TW;
     static constexpr int   LDS = LOADB ? N * PB : TW * PA;
     static constexpr int   CPT = KIND == KIND_C2C ? 1 : 2;
+    // column arithmetic by the multiply-and-shift constants alone (see div_col)
+    static constexpr bool  MG  = N >= 32;
     static_assert( LDS * (int)sizeof( C ) / ( SPLIT ? 2 : 1 ) <=
                        ( SHAPE >= 4 ? lds_limit_wide_bytes() : lds_limit_bytes() ),
                    "LDS tile too large" );
@@ -696,6 +705,10 @@ struct pass_kernel
         const int waves   = wgs * ( NT / 64 ) / 4;
         return waves < 1 ? 1 : ( waves > 4 ? 4 : waves );
     }();
+    // register-capped kernels form the output offsets again at the direct store instead of
+    // holding them through the stages (held, they were spilled to scratch: 8 bytes written
+    // per thread)
+    static constexpr bool OUT_LATE = MG && WAVES_PER_EU > 1;
 
     __device__ static __forceinline__ int L( int n, int c )
     {
@@ -916,6 +929,18 @@ struct pass_kernel
         }
         if ( direct )
         {
+            if constexpr ( OUT_LATE )
+            {
+                // an opaque copy of the column index keeps the compiler from reusing the
+                // offsets formed at the start
+                int c = x.c;
+                asm volatile( "" : "+v"( c ) );
+                const int64_t col = ( x.tile * TW + c ) * CPT;
+                int64_t       ia, ib;
+                col_offsets<MG>( p, col, ia, x.oa );
+                if ( CPT == 2 )
+                    col_offsets<MG>( p, col + 1, ib, x.ob );
+            }
             if constexpr ( PAIR_STORE )
             {
                 if ( p.vec & 2 )
@@ -996,9 +1021,9 @@ struct pass_kernel
         const bool    va  = col < p.ncols;
         const bool    vb  = CPT == 2 && col + 1 < p.ncols;
         int64_t       ia, oa, ib = 0, ob = 0;
-        col_offsets( p, col, ia, oa );
+        col_offsets<MG>( p, col, ia, oa );
         if ( CPT == 2 )
-            col_offsets( p, col + 1, ib, ob );
+            col_offsets<MG>( p, col + 1, ib, ob );
         if constexpr ( KIND == KIND_R2C )
         {
             // A[k] = (Z[k] + conj Z[N-k]) / 2 ;  B[k] = (Z[k] - conj Z[N-k]) / (2i)
@@ -1036,8 +1061,8 @@ struct pass_kernel
                     cmb = fmaxf( cmb, red[NT + ( cf ? t * TW + c2 : c2 * TPC + t )] );
                 const int ea = h16_exponent( cma ), eb = h16_exponent( cmb );
                 int64_t   xi, ya, yb = 0;
-                col_exponents( p, col, xi, ya );
-                col_exponents( p, col + 1, xi, yb );
+                col_exponents<MG>( p, col, xi, ya );
+                col_exponents<MG>( p, col + 1, xi, yb );
                 HE *out = static_cast<HE *>( p.out );
 #pragma unroll
                 for ( int i = 0; i < NI; ++i )
@@ -1087,7 +1112,7 @@ struct pass_kernel
                     lm = fmaxf( lm, cmag( zk[i] ) );
             const int e = h16_exponent( column_max( red, lm, c2, p.store_b != 0 ) * fabsf( (float)p.scale ) );
             int64_t   xi, ya;
-            col_exponents( p, col, xi, ya );
+            col_exponents<MG>( p, col, xi, ya );
             HE *out = static_cast<HE *>( p.out );
 #pragma unroll
             for ( int i = 0; i < NI; ++i )
@@ -1275,15 +1300,15 @@ struct pass_kernel
         x.va         = col < p.ncols;
         x.vb         = CPT == 2 && col + 1 < p.ncols;
         x.ib = x.ob = 0;
-        col_offsets( p, col, x.ia, x.oa );
+        col_offsets<MG>( p, col, x.ia, x.oa );
         if ( CPT == 2 )
-            col_offsets( p, col + 1, x.ib, x.ob );
+            col_offsets<MG>( p, col + 1, x.ib, x.ob );
         x.xa = x.xb = x.ya = x.yb = 0;
         if constexpr ( IOM != IO_F32 )
         {
-            col_exponents( p, col, x.xa, x.ya );
+            col_exponents<MG>( p, col, x.xa, x.ya );
             if ( CPT == 2 )
-                col_exponents( p, col + 1, x.xb, x.yb );
+                col_exponents<MG>( p, col + 1, x.xb, x.yb );
         }
 
         constexpr int R0   = RP.r[0];
@@ -1304,9 +1329,9 @@ struct pass_kernel
             {
                 const int64_t c  = x.tile * TW * 2 + l;
                 int64_t       io = 0, oo = 0, xi = 0, yo = 0;
-                col_offsets( p, c < p.ncols ? c : 0, io, oo );
+                col_offsets<MG>( p, c < p.ncols ? c : 0, io, oo );
                 if constexpr ( IOM & IO_H16_IN )
-                    col_exponents( p, c < p.ncols ? c : 0, xi, yo );
+                    col_exponents<MG>( p, c < p.ncols ? c : 0, xi, yo );
                 loff[l] = c < p.ncols ? io : -1;
                 xoff[l] = xi;
             }
@@ -1432,8 +1457,8 @@ struct pass_kernel
                 const int64_t     cf = x.tile * TW < p.ncols ? x.tile * TW : p.ncols - 1;
                 const int64_t     cl = cf + TW - 1 < p.ncols ? cf + TW - 1 : p.ncols - 1;
                 int64_t           xf = 0, xl = 0, yd = 0;
-                col_exponents( p, cf, xf, yd );
-                col_exponents( p, cl, xl, yd );
+                col_exponents<MG>( p, cf, xf, yd );
+                col_exponents<MG>( p, cl, xl, yd );
                 // the exponent loads are issued first and the data loads after them, all
                 // unconditionally (indices past the end read element 0, which exists)
                 constexpr int EP = ( 2 * N + NT - 1 ) / NT;
