@@ -7,6 +7,7 @@ Both are checked against the FP64 oracle, the fused chain also bitwise against
 forward + spectral_apply + backward, and the operators against closed forms."""
 
 import math
+import re
 
 import numpy as np
 import pytest
@@ -200,7 +201,10 @@ def test_filter_argument_errors(gpu):
 def test_filter_pass_tile_shapes_bitwise(gpu, tmp_path, fvar):
     """Every tile shape of the fused filter pass (0, 2, 4, in any tile order) gives bitwise
     the result of forward + spectral_apply + backward.  The shape is installed through a
-    tuning cache holding the handle's filter record."""
+    tuning cache holding the handle's filter record.  The cache also holds the separate
+    passes to radix schedule 0 (variant mod 32), the schedule the filter pass computes
+    with: the tuner may keep another schedule for a strided pass, which agrees with
+    schedule 0 to within rounding only."""
     import torch
     import fftm_amd
     dims = (512, 64, 32)
@@ -212,8 +216,15 @@ def test_filter_pass_tile_shapes_bitwise(gpu, tmp_path, fvar):
     key = [ln.split("=", 1)[0] for ln in lines if ln.endswith(tuple(f"={v}" for v in range(32)))
            and ln.split("=", 1)[0].endswith("_FILTER")]
     assert len(key) == 1, key
+    def rec(ln):
+        k, _, v = ln.partition("=")
+        if k == key[0]:
+            return f"{k}={fvar}"
+        if re.search(r"_(FWD|BWD)\d+$", k) and v.isdigit():
+            return f"{k}={int(v) % 32}"
+        return ln
     with open(cache, "w") as fh:
-        fh.write("\n".join(ln if not ln.startswith(key[0] + "=") else f"{key[0]}={fvar}" for ln in lines) + "\n")
+        fh.write("\n".join(rec(ln) for ln in lines) + "\n")
     assert f.load_tuning(cache) > 0
     g = torch.Generator(device="cuda").manual_seed(fvar)
     u = torch.rand(dims, generator=g, device="cuda") * 2 - 1
