@@ -54,6 +54,17 @@ def _b_alg(dims, elem_real):
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
 
 
+def _events(n, stream):
+    """n timing events, each recorded once on stream and waited for: torch creates the HIP
+    event at its first record, which would otherwise fall inside a timed region."""
+    import torch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    for e in ev:
+        e.record(stream)
+    torch.cuda.synchronize()
+    return ev
+
+
 def source_stamp():
     """Digest of the kernel sources (fftm_amd/csrc/*.hip, *.hpp): a traffic figure measured
     on another build of the kernels is stale."""
@@ -259,55 +270,74 @@ def run_gpu(dims, strategy, steps, warmup, check, tune=True, storage="fp32", u_d
         f.forward(u, X)
         f.backward(X, y)
 
-    for _ in range(warmup):
-        pair()
-    torch.cuda.synchronize()
-
     def run_one(bw, i):
         src, dst = (u, X) if bw == 0 else (X, y)
         _lib.check(L.fftm_mi_ffts_run_pass(h, bw, i, src.data_ptr(), dst.data_ptr(), 0), "run_pass")
 
-    def per_pass(nsteps):
+    # every timing event is created before the warmup, so that neither an event creation
+    # nor the synchronization after it leaves the GPU idle between the warmup and the
+    # timed region
+    ev_first = [_events(len(passes) + 1, stream) for _ in range(5)]
+    ev_pp = [_events(len(passes) + 1, stream) for _ in range(steps)]
+    ev0 = _events(steps, stream)
+    ev1 = _events(steps, stream)
+    pev = _events(steps + 1, stream)
+
+    # warmup: W pairs, then pairs back to back until 0.3 s have passed.  The GPU's clocks
+    # rise under sustained load: right after an idle gap of a second or more, the same
+    # pairs run 2-3 % slower for their first few tens of milliseconds.
+    tw = time.perf_counter()
+    for _ in range(warmup):
+        pair()
+    torch.cuda.synchronize()
+    while time.perf_counter() - tw < 0.3:
+        for _ in range(8):
+            pair()
+        torch.cuda.synchronize()
+
+    def per_pass(ev, stat=np.mean):
         # pairs pass by pass, a HIP event on the transform's stream after every pass
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(passes) + 1)] for _ in range(nsteps)]
-        for s in range(nsteps):
+        for s in range(len(ev)):
             ev[s][0].record(stream)
             for k, (bw, i) in enumerate(passes):
                 run_one(bw, i)
                 ev[s][k + 1].record(stream)
         torch.cuda.synchronize()
-        return {p: float(np.mean([ev[s][k].elapsed_time(ev[s][k + 1]) for s in range(nsteps)]))
+        return {p: float(stat([ev[s][k].elapsed_time(ev[s][k + 1]) for s in range(len(ev))]))
                 for k, p in enumerate(passes)}
 
-    # the dominant pass kernel, from three untimed instrumented pairs
-    first = per_pass(3)
+    # the dominant pass kernel, from five untimed instrumented pairs by the median
+    first = per_pass(ev_first, np.median)
     dom = max(passes, key=lambda p: first[p])
 
     # timed region: K pairs, pass by pass, with HIP events on the transform's stream
     # only around the dominant kernel (an event between every pass adds gaps)
-    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    def timed_pairs():
+        for s in range(steps):
+            for bw, i in passes:
+                if (bw, i) == dom:
+                    ev0[s].record(stream)
+                    run_one(bw, i)
+                    ev1[s].record(stream)
+                else:
+                    run_one(bw, i)
+
+    # the same K pairs once untimed first: measured with HIP events around the whole run,
+    # the first run of this sequence took 1.205 ms per 512^3 pair, the runs right after it
+    # 1.181 ms (10.43 against 10.15 ms at 1024^3), after the same warmup
+    timed_pairs()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(steps):
-        for bw, i in passes:
-            if (bw, i) == dom:
-                ev0[s].record(stream)
-                run_one(bw, i)
-                ev1[s].record(stream)
-            else:
-                run_one(bw, i)
+    timed_pairs()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     pair_s = (t1 - t0) / steps
     dom_ms = float(np.mean([ev0[s].elapsed_time(ev1[s]) for s in range(steps)]))
     # per-pass breakdown (untimed, events between all passes)
-    per_pass_ms = per_pass(steps)
+    per_pass_ms = per_pass(ev_pp)
     # also the plain calls (forward + backward API), with a HIP event between consecutive
     # pairs only: the reference's protocol (events around forward + backward,
     # test_benchmark_ffts_3D.cu:65-98) for the per-pair statistics
-    pev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-    torch.cuda.synchronize()
     t2 = time.perf_counter()
     pev[0].record(stream)
     for k in range(steps):
