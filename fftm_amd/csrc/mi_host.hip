@@ -973,8 +973,10 @@ int build_ffts_passes_h16( const ffts_layout &L, std::vector<pass_rt> &fwd, std:
     const int64_t extc[3] = { n0, n1, nzh };
     auto          align   = []( size_t b ) { return ( b + 255 ) & ~(size_t)255; };
     const size_t  hbytes  = align( (size_t)( n0 * n1 * nzh ) * ( split ? 8 : 4 ) );
-    const size_t  e0bytes = align( (size_t)std::max( n0 * n1, n1 * nzh ) );
-    const size_t  e1bytes = align( (size_t)( n0 * nzh ) );
+    // exponent arrays padded by 16 bytes: a strided pass reads a tile's exponents as whole
+    // dwords, up to 4 bytes past the last one
+    const size_t  e0bytes = align( (size_t)std::max( n0 * n1, n1 * nzh ) + 16 );
+    const size_t  e1bytes = align( (size_t)( n0 * nzh ) + 16 );
     const int64_t E0 = (int64_t)hbytes, E1 = (int64_t)( hbytes + e0bytes );
     work_bytes       = hbytes + e0bytes + e1bytes;
     int64_t yP0[3], yP1[3], yQ0[3], yQ1[3];

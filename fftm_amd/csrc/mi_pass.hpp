@@ -774,6 +774,17 @@ struct pass_kernel
     // written to LDS after the tile's loads are issued, so that no tile load waits for it.
     static constexpr int  TWPT = ( N + NT - 1 ) / NT;
     static constexpr bool TWR  = TWL && TWPT <= 4;
+    // Strided FP16 lines whose exponents run along the tile's columns stage the tile's
+    // exponents in LDS as dwords (XND per element index), where the extra LDS leaves the
+    // number of workgroups per CU unchanged, or at 3 or more (these kernels' registers
+    // allow at most 3 of 256 threads).  XBASE: the kernel's other LDS arrays.
+    static constexpr int  XND   = TW / 4 + 1;
+    static constexpr int  XBASE = LDS * (int)sizeof( LW ) + ( TWL ? N * (int)sizeof( C ) : 0 ) + 2 * N +
+                                  ( ( IOM & IO_H16_OUT ) ? 8 * NT : 0 );
+    static constexpr int  XWG0  = lds_limit_wide_bytes() / XBASE;
+    static constexpr int  XWG1  = lds_limit_wide_bytes() / ( XBASE + N * XND * 4 );
+    static constexpr bool XROW  = LOADB && KIND == KIND_C2C && ( IOM & IO_H16_IN ) && TW % 4 == 0 &&
+                                  ( XWG1 == XWG0 || XWG1 >= 3 );
 
     // ---- LDS exchange helpers: P = 2 moves whole values, P = 0 / 1 the real /
     // imaginary components (split exchange) ------------------------------------------
@@ -1513,6 +1524,86 @@ struct pass_kernel
                             else
                                 h = __builtin_bit_cast( HE, v[b][r].x );
                             v[b][r] = x.va ? h16_decode<SPL>( h, e ) : mk<float>( 0, 0 );
+                        }
+                        dft_fixed<R0, INV>( v[b], p.tw, N );
+                    }
+                }
+                loaded = true;
+            }
+        }
+        if constexpr ( XROW )
+        {
+            // strided FP16 lines whose exponent index runs along the columns (xs0 = 1: the x
+            // pass of the 3D layout, bin (x, z) of y-line exponent z + x xsn, and the inverse y
+            // pass).  When the tile's TW columns lie in one batch row, the exponents of its
+            // element n are TW consecutive bytes: the tile loads them as XND aligned dwords
+            // per element index into LDS, instead of one byte load per element.  The loads
+            // may read up to 4 bytes past a row's last exponent (the work area pads its
+            // exponent arrays).  Same values, same arithmetic.
+            const int64_t cf = x.tile * TW;
+            if ( !loaded && p.xs0 == 1 && ( reinterpret_cast<uintptr_t>( p.xin ) & 3 ) == 0 && cf + TW <= p.ncols &&
+                 cf % p.d0 + TW <= p.d0 )
+            {
+                __shared__ uint32_t xr[N * XND];
+                int64_t             xf = 0, yd = 0;
+                col_exponents<MG>( p, cf, xf, yd );
+                const uint32_t *xw = reinterpret_cast<const uint32_t *>( p.xin );
+                // the exponent loads are issued first and the data loads after them, all
+                // unconditionally (indices past the end read the tile's first dword)
+                constexpr int ITEMS = N * XND;
+                constexpr int EP    = ( ITEMS + NT - 1 ) / NT;
+                uint32_t      er[EP];
+#pragma unroll
+                for ( int i = 0; i < EP; ++i )
+                {
+                    const int     e  = tid + i * NT;
+                    const int     ec = ITEMS % NT == 0 || e < ITEMS ? e : 0;
+                    const int64_t a  = xf + (int64_t)( ec / XND ) * p.xsn;
+                    er[i]            = xw[( a >> 2 ) + ec % XND];
+                }
+                const HE *src = static_cast<const HE *>( p.in );
+#pragma unroll
+                for ( int b = 0; b < NB0; ++b )
+                {
+                    const int  j  = jmap<0>( x.t, b );
+                    const bool ok = NBF0 % TPC == 0 || j < NBF0;
+#pragma unroll
+                    for ( int r = 0; r < R0; ++r )
+                    {
+                        const HE raw = src[ok ? x.ia + (int64_t)( j + r * NBF0 ) * p.ies : 0];
+                        if constexpr ( SPL )
+                            v[b][r] = __builtin_bit_cast( C, raw );
+                        else
+                            v[b][r].x = __builtin_bit_cast( float, raw );
+                    }
+                }
+#pragma unroll
+                for ( int i = 0; i < EP; ++i )
+                {
+                    const int e = tid + i * NT;
+                    if ( ITEMS % NT == 0 || e < ITEMS )
+                        xr[e] = er[i];
+                }
+                __syncthreads();
+                const uint8_t *xb = reinterpret_cast<const uint8_t *>( xr );
+#pragma unroll
+                for ( int b = 0; b < NB0; ++b )
+                {
+                    const int j = jmap<0>( x.t, b );
+                    if ( NBF0 % TPC == 0 || j < NBF0 )
+                    {
+#pragma unroll
+                        for ( int r = 0; r < R0; ++r )
+                        {
+                            const int     n = j + r * NBF0;
+                            const int64_t a = xf + (int64_t)n * p.xsn;
+                            const int     e = (int8_t)xb[n * XND * 4 + (int)( a & 3 ) + x.c];
+                            HE            h;
+                            if constexpr ( SPL )
+                                h = __builtin_bit_cast( HE, v[b][r] );
+                            else
+                                h = __builtin_bit_cast( HE, v[b][r].x );
+                            v[b][r] = h16_decode<SPL>( h, e );
                         }
                         dft_fixed<R0, INV>( v[b], p.tw, N );
                     }

@@ -209,11 +209,13 @@ def test_ffts_validate_fp16_storage():
     nzh = n2 // 2 + 1
     rc = L.lib().fftm_mi_ffts_validate(0, L.i64_array([n0, n1, n2]), L.F32_H16, 0, ctypes.byref(ws))
     assert rc == 0, L.last_error()
-    assert ws.value == al(4 * n0 * n1 * nzh) + al(max(n0 * n1, n1 * nzh)) + al(n0 * nzh)
+    # FP16 bins, then the two exponent arrays, each padded by 16 bytes (a strided pass
+    # reads a tile's exponents as whole dwords, up to 4 bytes past the last one)
+    assert ws.value == al(4 * n0 * n1 * nzh) + al(max(n0 * n1, n1 * nzh) + 16) + al(n0 * nzh + 16)
     # hi + lo FP16 storage: 8 bytes per stored bin
     rc = L.lib().fftm_mi_ffts_validate(0, L.i64_array([n0, n1, n2]), L.F32_H16X2, 0, ctypes.byref(ws))
     assert rc == 0, L.last_error()
-    assert ws.value == al(8 * n0 * n1 * nzh) + al(max(n0 * n1, n1 * nzh)) + al(n0 * nzh)
+    assert ws.value == al(8 * n0 * n1 * nzh) + al(max(n0 * n1, n1 * nzh) + 16) + al(n0 * nzh + 16)
     # 4D layouts and lengths outside 128..2048 are not implemented for FP16 storage
     assert L.lib().fftm_mi_ffts_validate(1, L.i64_array([128] * 4), L.F32_H16, 0, ctypes.byref(ws)) == 16
     assert L.lib().fftm_mi_ffts_validate(0, L.i64_array([128, 96, 128]), L.F32_H16, 0, ctypes.byref(ws)) == 16
