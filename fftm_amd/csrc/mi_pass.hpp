@@ -1353,29 +1353,76 @@ struct pass_kernel
             constexpr int ITEMS = 2 * TW * NH;
             constexpr int PER   = ( ITEMS + NT - 1 ) / NT;
             C             zs[PER];
-#pragma unroll
-            for ( int i = 0; i < PER; ++i )
+            bool          xrow = false;
+            if constexpr ( IOM & IO_H16_IN )
             {
-                const int     e    = tid + i * NT;
-                const bool    in   = ITEMS % NT == 0 || e < ITEMS;
-                const int     ec   = in ? e : ITEMS - 1;
-                const int     line = ec / NH, k = ec % NH;
-                const int64_t lo   = loff[line];
-                const bool    ok   = in && lo >= 0;
-                const int64_t src  = ok ? lo + (int64_t)k * p.ies : 0;
-                C             z;
-                if constexpr ( IOM & IO_H16_IN )
-                    z = h16_load<SPL>( p.in, src, p.xin, ok ? xoff[line] + (int64_t)k * p.xsn : 0 );
-                else
-                    z = static_cast<const C *>( p.in )[src];
-                zs[i] = ok ? z : mk<T>( 0, 0 );
+                // FP16 lines whose exponent index runs along the line (xsn = 1) and does not
+                // depend on the fastest batch index (xs0 = 0: the C2R pass of the 3D layout,
+                // bin (x, y, z) of y-line exponent z + x xs2).  When the tile's lines lie in
+                // one batch row they share one row of NH exponents, loaded once per tile as
+                // XD2 dwords into LDS (up to 7 bytes past the row: the work area pads its
+                // exponent arrays), instead of one byte load per bin.  Same values, same
+                // arithmetic.
+                constexpr int       XD2 = ( NH + 3 ) / 4 + 1;
+                __shared__ uint32_t xr2[XD2];
+                const int64_t       c0 = x.tile * TW * 2;
+                xrow = p.xsn == 1 && p.xs0 == 0 && ( reinterpret_cast<uintptr_t>( p.xin ) & 3 ) == 0 &&
+                       c0 + 2 * TW <= p.ncols && c0 % p.d0 + 2 * TW <= p.d0;
+                if ( xrow )
+                {
+                    const int64_t   xf = xoff[0];
+                    const uint32_t *xw = reinterpret_cast<const uint32_t *>( p.xin );
+                    uint32_t        er = 0;
+                    if ( tid < XD2 )
+                        er = xw[( xf >> 2 ) + tid];
+                    const HE *srch = static_cast<const HE *>( p.in );
+                    HE        raw[PER];
+#pragma unroll
+                    for ( int i = 0; i < PER; ++i )
+                    {
+                        const int e  = tid + i * NT;
+                        const int ec = ITEMS % NT == 0 || e < ITEMS ? e : ITEMS - 1;
+                        raw[i]       = srch[loff[ec / NH] + (int64_t)( ec % NH ) * p.ies];
+                    }
+                    if ( tid < XD2 )
+                        xr2[tid] = er;
+                    __syncthreads();
+                    const uint8_t *xb = reinterpret_cast<const uint8_t *>( xr2 ) + ( xf & 3 );
+#pragma unroll
+                    for ( int i = 0; i < PER; ++i )
+                    {
+                        const int e = tid + i * NT;
+                        if ( ITEMS % NT == 0 || e < ITEMS )
+                            st[e] = h16_decode<SPL>( raw[i], (int8_t)xb[e % NH] ); // line * NH + k == e
+                    }
+                }
             }
-#pragma unroll
-            for ( int i = 0; i < PER; ++i )
+            if ( !xrow )
             {
-                const int e = tid + i * NT;
-                if ( ITEMS % NT == 0 || e < ITEMS )
-                    st[e] = zs[i]; // line * NH + k == e
+#pragma unroll
+                for ( int i = 0; i < PER; ++i )
+                {
+                    const int     e    = tid + i * NT;
+                    const bool    in   = ITEMS % NT == 0 || e < ITEMS;
+                    const int     ec   = in ? e : ITEMS - 1;
+                    const int     line = ec / NH, k = ec % NH;
+                    const int64_t lo   = loff[line];
+                    const bool    ok   = in && lo >= 0;
+                    const int64_t src  = ok ? lo + (int64_t)k * p.ies : 0;
+                    C             z;
+                    if constexpr ( IOM & IO_H16_IN )
+                        z = h16_load<SPL>( p.in, src, p.xin, ok ? xoff[line] + (int64_t)k * p.xsn : 0 );
+                    else
+                        z = static_cast<const C *>( p.in )[src];
+                    zs[i] = ok ? z : mk<T>( 0, 0 );
+                }
+#pragma unroll
+                for ( int i = 0; i < PER; ++i )
+                {
+                    const int e = tid + i * NT;
+                    if ( ITEMS % NT == 0 || e < ITEMS )
+                        st[e] = zs[i]; // line * NH + k == e
+                }
             }
             __syncthreads();
 #pragma unroll
