@@ -775,16 +775,23 @@ struct pass_kernel
     static constexpr int  TWPT = ( N + NT - 1 ) / NT;
     static constexpr bool TWR  = TWL && TWPT <= 4;
     // Strided FP16 lines whose exponents run along the tile's columns stage the tile's
-    // exponents in LDS as dwords (XND per element index), where the extra LDS leaves the
-    // number of workgroups per CU unchanged, or at 3 or more (these kernels' registers
-    // allow at most 3 of 256 threads).  XBASE: the kernel's other LDS arrays.
-    static constexpr int  XND   = TW / 4 + 1;
-    static constexpr int  XBASE = LDS * (int)sizeof( LW ) + ( TWL ? N * (int)sizeof( C ) : 0 ) + 2 * N +
-                                  ( ( IOM & IO_H16_OUT ) ? 8 * NT : 0 );
-    static constexpr int  XWG0  = lds_limit_wide_bytes() / XBASE;
-    static constexpr int  XWG1  = lds_limit_wide_bytes() / ( XBASE + N * XND * 4 );
-    static constexpr bool XROW  = LOADB && KIND == KIND_C2C && ( IOM & IO_H16_IN ) && TW % 4 == 0 &&
-                                  ( XWG1 == XWG0 || XWG1 >= 3 );
+    // exponents in LDS, loaded as dwords (XND per element index): the dwords themselves
+    // (XDW), or only the N x TW exponent bytes where the dwords would take LDS that a
+    // workgroup needs.  Either is used where the extra LDS leaves the number of workgroups
+    // per CU unchanged, or at 3 or more (these kernels' registers allow at most 3 of 256
+    // threads).  XBASE: the kernel's other LDS arrays.
+    static constexpr int XND   = TW / 4 + 1;
+    static constexpr int XBASE = LDS * (int)sizeof( LW ) + ( TWL ? N * (int)sizeof( C ) : 0 ) + 2 * N +
+                                 ( ( IOM & IO_H16_OUT ) ? 8 * NT : 0 );
+    static constexpr bool x_lds_ok( int extra )
+    {
+        const int wg0 = lds_limit_wide_bytes() / XBASE, wg1 = lds_limit_wide_bytes() / ( XBASE + extra );
+        return wg1 == wg0 || wg1 >= 3;
+    }
+    static constexpr bool XDW  = x_lds_ok( N * XND * 4 );
+    static constexpr bool XROW = LOADB && KIND == KIND_C2C && ( IOM & IO_H16_IN ) && TW % 4 == 0 &&
+                                 ( XDW || x_lds_ok( N * TW ) );
+    static constexpr int  XLDS = XDW ? N * XND * 4 : N * TW;
 
     // ---- LDS exchange helpers: P = 2 moves whole values, P = 0 / 1 the real /
     // imaginary components (split exchange) ------------------------------------------
@@ -1584,14 +1591,16 @@ struct pass_kernel
             // pass of the 3D layout, bin (x, z) of y-line exponent z + x xsn, and the inverse y
             // pass).  When the tile's TW columns lie in one batch row, the exponents of its
             // element n are TW consecutive bytes: the tile loads them as XND aligned dwords
-            // per element index into LDS, instead of one byte load per element.  The loads
-            // may read up to 4 bytes past a row's last exponent (the work area pads its
-            // exponent arrays).  Same values, same arithmetic.
+            // per element index and keeps the TW bytes of each in LDS, instead of one byte
+            // load per element.  The loads may read up to 4 bytes past a row's last
+            // exponent (the work area pads its exponent arrays).  Same values, same
+            // arithmetic.
             const int64_t cf = x.tile * TW;
             if ( !loaded && p.xs0 == 1 && ( reinterpret_cast<uintptr_t>( p.xin ) & 3 ) == 0 && cf + TW <= p.ncols &&
                  cf % p.d0 + TW <= p.d0 )
             {
-                __shared__ uint32_t xr[N * XND];
+                __shared__ uint32_t xr32[XLDS / 4];
+                uint8_t            *xr = reinterpret_cast<uint8_t *>( xr32 );
                 int64_t             xf = 0, yd = 0;
                 col_exponents<MG>( p, cf, xf, yd );
                 const uint32_t *xw = reinterpret_cast<const uint32_t *>( p.xin );
@@ -1629,10 +1638,23 @@ struct pass_kernel
                 {
                     const int e = tid + i * NT;
                     if ( ITEMS % NT == 0 || e < ITEMS )
-                        xr[e] = er[i];
+                    {
+                        if constexpr ( XDW )
+                            xr32[e] = er[i];
+                        else
+                        {
+                            // byte q of dword k of row n is exponent (a & ~3) + 4 k + q of the
+                            // row, column 4 k + q - (a & 3) of the tile
+                            const int n  = e / XND;
+                            const int c0 = 4 * ( e % XND ) - (int)( ( xf + (int64_t)n * p.xsn ) & 3 );
+#pragma unroll
+                            for ( int q = 0; q < 4; ++q )
+                                if ( c0 + q >= 0 && c0 + q < TW )
+                                    xr[n * TW + c0 + q] = (uint8_t)( er[i] >> ( 8 * q ) );
+                        }
+                    }
                 }
                 __syncthreads();
-                const uint8_t *xb = reinterpret_cast<const uint8_t *>( xr );
 #pragma unroll
                 for ( int b = 0; b < NB0; ++b )
                 {
@@ -1642,9 +1664,12 @@ struct pass_kernel
 #pragma unroll
                         for ( int r = 0; r < R0; ++r )
                         {
-                            const int     n = j + r * NBF0;
-                            const int64_t a = xf + (int64_t)n * p.xsn;
-                            const int     e = (int8_t)xb[n * XND * 4 + (int)( a & 3 ) + x.c];
+                            const int n = j + r * NBF0;
+                            int       e;
+                            if constexpr ( XDW )
+                                e = (int8_t)xr[n * XND * 4 + (int)( ( xf + (int64_t)n * p.xsn ) & 3 ) + x.c];
+                            else
+                                e = (int8_t)xr[n * TW + x.c];
                             HE            h;
                             if constexpr ( SPL )
                                 h = __builtin_bit_cast( HE, v[b][r] );
