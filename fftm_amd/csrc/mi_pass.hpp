@@ -778,8 +778,8 @@ struct pass_kernel
     // exponents in LDS, loaded as dwords (XND per element index): the dwords themselves
     // (XDW), or only the N x TW exponent bytes where the dwords would take LDS that a
     // workgroup needs.  Either is used where the extra LDS leaves the number of workgroups
-    // per CU unchanged, or at 3 or more (these kernels' registers allow at most 3 of 256
-    // threads).  XBASE: the kernel's other LDS arrays.
+    // per CU unchanged, or at 3 or more (register use holds these kernels to 3 or fewer
+    // workgroups per CU).  XBASE: the kernel's other LDS arrays.
     static constexpr int XND   = TW / 4 + 1;
     static constexpr int XBASE = LDS * (int)sizeof( LW ) + ( TWL ? N * (int)sizeof( C ) : 0 ) + 2 * N +
                                  ( ( IOM & IO_H16_OUT ) ? 8 * NT : 0 );
